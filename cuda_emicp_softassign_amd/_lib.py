@@ -1,0 +1,90 @@
+"""ctypes binding of libemsa.so (the C ABI declared in include/emsa.h).
+
+The library must be built in-tree (``python -m cuda_emicp_softassign_amd.build``
+or ``__graft_entry__.build()``). There is no fallback: a missing library
+raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libemsa.so")
+
+EMSA_OK = 0
+EMSA_EINVAL = -1
+EMSA_ENOMEM = -2
+EMSA_EHIP = -3
+EMSA_ENONFINITE = -4
+EMSA_ENODEV = -5
+SOFTASSIGN_MATERIALISED = 0
+SOFTASSIGN_LOGDOMAIN = 1
+TRACE_W = 32
+
+
+class emsa_params(C.Structure):
+    """Mirror of struct emsa_params (include/emsa.h)."""
+    _fields_ = [
+        ("JMAX", C.c_int), ("I0", C.c_int), ("I1", C.c_int),
+        ("alpha", C.c_float), ("T_0", C.c_float), ("TFACTOR", C.c_float),
+        ("moutlier", C.c_float),
+        ("sigma_p2", C.c_float), ("sigma_inf", C.c_float), ("sigma_factor", C.c_float),
+        ("d_02", C.c_float),
+        ("maxIteration", C.c_int), ("noviewer", C.c_int), ("nostop", C.c_int),
+        ("notimer", C.c_int),
+    ]
+
+
+# Every function the header declares, with its ctypes signature.
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_pp = C.POINTER(emsa_params)
+SIGNATURES = {
+    "emsa_default_params": (None, [_pp]),
+    "emsa_last_error": (C.c_char_p, []),
+    "emsa_emicp_iterations": (C.c_int, [_pp]),
+    "emsa_emicp": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp]),
+    "emsa_emicp_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, _dp, C.c_int]),
+    "emsa_emicp_step": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float, C.c_float,
+                                  _dp, _dp, _dp]),
+    "emsa_softassign": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int]),
+    "emsa_softassign_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int,
+                                         _dp, C.c_int]),
+    "emsa_find_rt_from_s": (C.c_int, [_fp, _fp, _fp, _fp, _fp]),
+    "emsa_emicp_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, _pp,
+                                 C.c_void_p]),
+}
+
+_lib = None
+
+
+class EmsaError(RuntimeError):
+    """A libemsa entry point returned a negative status."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__("%s failed (%d): %s" % (fn, code, msg))
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libemsa.so is not built (%s); run python -m cuda_emicp_softassign_amd.build"
+                % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(fn, rc):
+    if rc < 0:
+        msg = load().emsa_last_error()
+        raise EmsaError(fn, rc, msg.decode() if msg else "")
+    return rc

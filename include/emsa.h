@@ -1,0 +1,123 @@
+/*
+ * emsa -- MI355X-native soft-correspondence rigid registration (EM-ICP and
+ * Softassign), C ABI.
+ *
+ * This is the thin boundary under the C++ drop-in header 3dregistration.h.
+ * Plain pointers and sizes only. Point clouds are SoA float arrays
+ * [x0..x(n-1), y0.., z0..], exactly the layout cloud2data builds
+ * (src/cloud2data.cpp:29-47). R is 3x3 row-major (r11 r12 r13 r21 ...),
+ * t is (tx, ty, tz); both are in/out: on entry the initial estimate, on exit
+ * the result (src/3dregistration.h:82-99, README.md:206-214). Every algorithm
+ * solves min ||X - (R Y + t)|| with X the target (cloud_target) and Y the
+ * source (cloud_source).
+ *
+ * Every entry point returns EMSA_OK (0) or a negative code. On failure R and
+ * t are left unchanged and emsa_last_error() describes the failure. Calls
+ * are serialised per device; the library never tears down the caller's HIP
+ * context (unlike cudaThreadExit in src/emicp.cu:713).
+ */
+#ifndef EMSA_H_
+#define EMSA_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    EMSA_OK = 0,
+    EMSA_EINVAL = -1,    /* bad argument (sizes, NULL pointers, I1 < 1, ...) */
+    EMSA_ENOMEM = -2,    /* device allocation failed */
+    EMSA_EHIP = -3,      /* HIP runtime error */
+    EMSA_ENONFINITE = -4,/* non-finite input or result */
+    EMSA_ENODEV = -5     /* no HIP device */
+};
+
+enum {
+    EMSA_SOFTASSIGN_MATERIALISED = 0, /* FP32 (N x M) kernel matrix in HBM */
+    EMSA_SOFTASSIGN_LOGDOMAIN = 1     /* potentials, affinities recomputed per pass */
+};
+
+/* Numeric fields of registrationParameters (src/3dregistration.h:39-65), same
+ * order and meaning. Viewer members, argc/argv are host-side only. */
+typedef struct emsa_params {
+    int JMAX;         /* annealing temperatures (softassign), default 100 */
+    int I0;           /* inner iterations per temperature, default 5 */
+    int I1;           /* Sinkhorn sweeps per inner iteration, default 3 */
+    float alpha;      /* outlier parameter, default 3 */
+    float T_0;        /* initial temperature, default 100 */
+    float TFACTOR;    /* temperature factor, default 0.95 */
+    float moutlier;   /* slack row/column value, default expf(-1)/sqrtf(T_0) */
+    float sigma_p2;   /* EM-ICP initial sigma^2, default 0.01 */
+    float sigma_inf;  /* EM-ICP final sigma^2, default 1e-5 */
+    float sigma_factor; /* EM-ICP sigma^2 factor, default 0.9 */
+    float d_02;       /* EM-ICP outlier distance^2, default 0.01 */
+    int maxIteration; /* ICP only (not provided by this library), default 30 */
+    int noviewer;
+    int nostop;
+    int notimer;
+} emsa_params;
+
+/* Fills the defaults of src/main.cpp:280-315. */
+void emsa_default_params(emsa_params *p);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *emsa_last_error(void);
+
+/* Number of sigma^2 steps the EM-ICP loop runs for these parameters, computed
+ * in float exactly as the reference loop (src/emicp.cu:443, :668). */
+int emsa_emicp_iterations(const emsa_params *p);
+
+/* ---- Host-buffer entry points (the drop-in boundary) -------------------- */
+
+/* EM-ICP on the GPU. Replaces emicp() (src/emicp.cu:247-716; declared at
+ * src/3dregistration.h:87-90). X: N target points, Y: M source points. */
+int emsa_emicp(const float *X, int N, const float *Y, int M, float *R, float *t,
+               const emsa_params *p);
+
+/* Same as emsa_emicp, also writing one record of EMSA_TRACE_W doubles per
+ * sigma step into trace (capacity trace_cap records): sigma_p2, Lambda,
+ * Xc[3], Yc[3], S[9] (S[i+3j], i target side, j source side), R[9], t[3],
+ * residual. Returns the number of steps written, or a negative code. */
+#define EMSA_TRACE_W 32
+int emsa_emicp_traced(const float *X, int N, const float *Y, int M, float *R, float *t,
+                      const emsa_params *p, double *trace, int trace_cap);
+
+/* One sigma^2 step from (R, t): the per-source-row normalisers
+ * C_r = sum_c A_rc + exp(-d_02/sigma_p2) and lambda_r = sum_c A'_rc
+ * (src/emicp.cu:477-519), the trace record of the step, and R, t updated.
+ * C and lambda are double arrays of M entries (either may be NULL). */
+int emsa_emicp_step(const float *X, int N, const float *Y, int M, float *R, float *t,
+                    float sigma_p2, float d_02, double *C, double *lambda, double *record);
+
+/* Softassign on the GPU. Replaces softassign() (src/softassign.cu:256-886;
+ * declared at src/3dregistration.h:82-86). variant: EMSA_SOFTASSIGN_*. */
+int emsa_softassign(const float *X, int N, const float *Y, int M, float *R, float *t,
+                    const emsa_params *p, int variant);
+
+/* As emsa_softassign with one EMSA_TRACE_W record per inner iteration:
+ * T, sumM, Xc[3], Yc[3], S[9], R[9], t[3]. */
+int emsa_softassign_traced(const float *X, int N, const float *Y, int M, float *R, float *t,
+                           const emsa_params *p, int variant, double *trace, int trace_cap);
+
+/* Horn's closed-form Procrustes step, computed on the device in FP64.
+ * Replaces findRTfromS (src/findRTfromS.cpp:33-97; declared at
+ * src/3dregistration.h:104-107). S is h_S of the reference: S[3a+b] =
+ * sum (source)_a (target)_b. */
+int emsa_find_rt_from_s(const float *Xc, const float *Yc, const float *S, float *R, float *t);
+
+/* ---- Device-buffer entry points (inputs resident in HBM) ---------------- */
+
+/* EM-ICP with X, Y in device memory (SoA, unpadded) and Rt a device array of
+ * 12 floats (R row-major, then t), in/out. stream: a hipStream_t or NULL for
+ * the null stream. Asynchronous with respect to the host; errors detected at
+ * launch are returned. */
+int emsa_emicp_dev(const float *dX, int N, const float *dY, int M, float *dRt,
+                   const emsa_params *p, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMSA_H_ */
