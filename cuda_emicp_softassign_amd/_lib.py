@@ -70,6 +70,13 @@ class EmsaError(RuntimeError):
 def load():
     global _lib
     if _lib is None:
+        # torch bundles its own libamdhip64.so.7. Import it first so that this
+        # library binds to the same HIP runtime (same soname) instead of
+        # loading a second one from /opt/rocm into the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 "libemsa.so is not built (%s); run python -m cuda_emicp_softassign_amd.build"

@@ -70,6 +70,10 @@ __global__ void emicp_output_kernel(const double* __restrict__ state, float* __r
     if (threadIdx.x < 12) Rt[threadIdx.x] = (float)state[threadIdx.x];
 }
 
+// COMP: subtract the rounding residual of p_r in every difference,
+// d = (x - p_hi) - p_lo, where x - p_hi is exact for the pairs that matter
+// (Sterbenz), so d carries no error from rounding p_r to FP32.
+template <bool COMP>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, int N, int M,
                   int ldx, int chunks, int waveCols, float negk, float sigma_p2, double outlier,
@@ -101,6 +105,8 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
     const double pyd = Rm[3] * yx + Rm[4] * yy + Rm[5] * yz + tv[1];
     const double pzd = Rm[6] * yx + Rm[7] * yy + Rm[8] * yz + tv[2];
     const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
+    const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
+                plz = (float)(pzd - (double)pz);
 
     const float* __restrict__ Xx = Xpad;
     const float* __restrict__ Xy = Xpad + ldx;
@@ -113,7 +119,12 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
 #pragma unroll
         for (int u = 0; u < kColGroup; ++u) {
             const float xx = Xx[c + u], xy = Xy[c + u], xz = Xz[c + u];
-            const float dx = xx - px, dy = xy - py, dz = xz - pz;
+            float dx = xx - px, dy = xy - py, dz = xz - pz;
+            if (COMP) {
+                dx -= plx;
+                dy -= ply;
+                dz -= plz;
+            }
             float s = dx * dx;
             s = fmaf(dy, dy, s);
             s = fmaf(dz, dz, s);
@@ -326,8 +337,8 @@ int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const bool last = (it + 1 == sigmas.size());
-        hipLaunchKernelGGL(emicp_iter_kernel, dim3((unsigned)nblocks), dim3(kEmRows * kEmWaves),
-                           0, s, ctx.Xpad.as<float>(), dY, N, M, pl.ldx, pl.chunks, pl.waveCols,
+        hipLaunchKernelGGL(emicp_iter_kernel<true>, dim3((unsigned)nblocks),
+                           dim3(kEmRows * kEmWaves), 0, s, ctx.Xpad.as<float>(), dY, N, M, pl.ldx, pl.chunks, pl.waveCols,
                            negk, sig, outlier, floorA, state, ctx.partial.as<float>(),
                            ctx.moments.as<double>(), ctx.counters.as<unsigned>(), trace,
                            last ? rows : nullptr, (int)it);

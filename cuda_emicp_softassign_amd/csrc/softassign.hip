@@ -460,10 +460,11 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         if ((rc = ctx.K.reserve(sizeof(float) * (size_t)N * pl.ldk)) != EMSA_OK) return rc;
         b.K = ctx.K.as<float>();
     }
-    if ((rc = ctx.v.reserve(sizeof(float) * ((size_t)N + 2 * (size_t)pl.ldk))) != EMSA_OK)
-        return rc;
+    // u, v, bcol each start on a 256-byte boundary (v and bcol are read as float4)
+    const size_t npad = ((size_t)N + 63) / 64 * 64;
+    if ((rc = ctx.v.reserve(sizeof(float) * (npad + 2 * (size_t)pl.ldk))) != EMSA_OK) return rc;
     b.u = ctx.v.as<float>();
-    b.v = b.u + N;
+    b.v = b.u + npad;
     b.bcol = b.v + pl.ldk;
     if ((rc = ctx.orow.reserve(sizeof(float) * ((size_t)N + M))) != EMSA_OK) return rc;
     b.orow = ctx.orow.as<float>();

@@ -1,0 +1,95 @@
+"""Softassign on the GPU (both variants) against the FP64 oracle.
+
+The reference has no CPU Softassign, so parity is against the FP64 oracle of
+the identical iteration (oracle/emsa_oracle_impl.h, literal materialised
+matrix with in-place divisions). Tolerance: R within 1e-4 rad per inner
+iteration (asserted 2e-5), t within 1e-5 of the cloud extent.
+"""
+import numpy as np
+import pytest
+
+from cuda_emicp_softassign_amd.clouds import geodesic, rotation, surface_with_outliers
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["materialised", "logdomain"]
+
+
+@pytest.fixture(scope="module")
+def E():
+    import cuda_emicp_softassign_amd as E
+    E._lib.load()
+    return E
+
+
+def _case(seed=11, n=300, m=280):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-0.5, 0.5, size=(n, 3))
+    Y = (X[:m] - 0.03) @ rotation([1, 1, 0], 0.2) + rng.normal(0, 0.002, size=(m, 3))
+    return X.astype(np.float32), Y.astype(np.float32)
+
+
+def _compare(E, oracle, X, Y, variant, tol=1e-4, **kw):
+    p = E.registrationParameters(**kw)
+    Ro, to, tro = oracle.softassign(X.astype(np.float64), Y.astype(np.float64),
+                                    JMAX=p.JMAX, I0=p.I0, I1=p.I1, alpha=p.alpha, T_0=p.T_0,
+                                    TFACTOR=p.TFACTOR, moutlier=p.moutlier)
+    R, t, tr = E.softassign_traced(X, Y, np.eye(3), np.zeros(3), p, variant=variant)
+    assert len(tr) == len(tro) == p.JMAX * p.I0
+    for k in range(len(tr)):
+        assert geodesic(tr[k, 17:26].reshape(3, 3), tro[k, 17:26].reshape(3, 3)) < tol, k
+        assert abs(tr[k, 1] - tro[k, 1]) <= 1e-4 * abs(tro[k, 1]), k
+    extent = np.ptp(X, axis=0).max()
+    assert geodesic(R.astype(np.float64), Ro) < tol
+    assert np.abs(t - to).max() < 1e-5 * extent
+    return R, t, tr
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_short_schedule_matches_oracle(E, oracle, variant):
+    X, Y = _case()
+    _compare(E, oracle, X, Y, variant, JMAX=30, I0=5, I1=3, T_0=0.5, TFACTOR=0.9, alpha=0.05)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_default_parameters_match_oracle(E, oracle, variant):
+    """Reference defaults (JMAX 100, I0 5, I1 3, alpha 3, T_0 100, 0.95)."""
+    X, Y = _case(seed=4, n=257, m=190)
+    _compare(E, oracle, X, Y, variant)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("N,M", [(4, 7), (7, 4), (5, 300), (300, 5), (257, 513)])
+def test_ragged_sizes(E, oracle, variant, N, M):
+    """Uneven and tiny sizes; Y is a moved, noisy resampling of X so that
+    Horn's problem is well posed (for unrelated clouds the top eigenvalue of
+    Horn's matrix is nearly degenerate and any rounding picks another one)."""
+    rng = np.random.default_rng(N + 31 * M)
+    X = rng.uniform(-0.5, 0.5, size=(N, 3))
+    Y = (X[rng.integers(0, N, size=M)] - 0.02) @ rotation([1, 0, 1], 0.15)
+    Y = Y + rng.normal(0, 0.005, size=Y.shape)
+    _compare(E, oracle, X.astype(np.float32), Y.astype(np.float32), variant, JMAX=8, I0=2,
+             I1=2, T_0=1.0, TFACTOR=0.9)
+
+
+def test_variants_agree_full_size(E):
+    """N = M = 5032 surface + 10% outliers, reference defaults: the
+    materialised and log-domain variants give the same registration."""
+    X, Y, Rt, tt = surface_with_outliers(5032, seed=1)
+    Rm, tm = E.softassign(X, Y, np.eye(3), np.zeros(3), variant="materialised")
+    Rl, tl = E.softassign(X, Y, np.eye(3), np.zeros(3), variant="logdomain")
+    assert geodesic(Rm.astype(np.float64), Rl.astype(np.float64)) < 1e-4
+    assert np.abs(tm - tl).max() < 1e-5
+    assert abs(np.linalg.det(Rm.astype(np.float64)) - 1) < 1e-5
+
+
+def test_invalid_and_empty_schedules(E):
+    X, Y = _case(n=50, m=40)
+    hR = np.eye(3, dtype=np.float32)
+    ht = np.zeros(3, np.float32)
+    with pytest.raises(E.EmsaError) as ei:
+        E.softassign(X, Y, hR, ht, E.registrationParameters(I1=0))
+    assert ei.value.code == E._lib.EMSA_EINVAL
+    R0 = rotation([0, 0, 1], 0.4).astype(np.float32)
+    R, t = E.softassign(X, Y, R0.copy(), ht.copy(), E.registrationParameters(JMAX=0))
+    assert np.array_equal(R, R0) and not t.any()
