@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
 FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 1 (B), 6 (W fma)
+VALU_PER_PAIR = 15            # issued per pair: 6 sub (compensated), 3 (d^2), 1 scale, 5 accumulate
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 
 
@@ -134,6 +135,18 @@ def main():
                            % (FLOPS_PER_PAIR, pairs),
         },
     }
+
+    # binding compute ceiling of the fused affinity, from probes of this device
+    import ctypes
+    pe, pf = ctypes.c_double(), ctypes.c_double()
+    E._lib.check("probe", E._lib.load().emsa_probe_rates(ctypes.byref(pe), ctypes.byref(pf)))
+    pair_rate = pairs / (step_ms * 1e-3)
+    ceiling = 1.0 / (VALU_PER_PAIR / pf.value + 1.0 / pe.value)  # shared issue slots
+    out["roofline"].update({
+        "exp_rate_per_s": pe.value, "fma_rate_per_s": pf.value,
+        "valu_per_pair": VALU_PER_PAIR, "pair_rate_per_s": pair_rate,
+        "pair_ceiling_per_s": ceiling, "frac_of_binding_ceiling": pair_rate / ceiling,
+    })
 
     if not args.no_check:
         from oracle import oracle as O

@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -47,7 +47,7 @@ class registrationParameters:
     alpha: float = 3.0
     T_0: float = 100.0
     TFACTOR: float = 0.95
-    moutlier: float = field(default_factory=_default_moutlier)
+    moutlier: float = None  # None: 1/sqrtf(T_0)*expf(-1), from this T_0 (main.cpp:286)
     sigma_p2: float = 0.01
     sigma_inf: float = 0.00001
     sigma_factor: float = 0.9
@@ -56,6 +56,10 @@ class registrationParameters:
     noviewer: int = 1
     nostop: int = 0
     notimer: int = 1
+
+    def __post_init__(self):
+        if self.moutlier is None:
+            self.moutlier = _default_moutlier(self.T_0)
 
     def to_c(self) -> emsa_params:
         p = emsa_params()

@@ -54,6 +54,7 @@ SIGNATURES = {
     "emsa_find_rt_from_s": (C.c_int, [_fp, _fp, _fp, _fp, _fp]),
     "emsa_emicp_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, _pp,
                                  C.c_void_p]),
+    "emsa_probe_rates": (C.c_int, [_dp, _dp]),
 }
 
 _lib = None

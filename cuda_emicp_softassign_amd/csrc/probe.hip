@@ -1,0 +1,86 @@
+// Throughput probes for the compute ceilings the registration kernels are
+// measured against: the v_exp_f32 (transcendental) issue rate and the FP32
+// FMA issue rate of the whole device. Each lane runs 8 independent
+// dependency chains so that latency is hidden and issue rate is measured.
+#include "emsa_device.h"
+#include "emsa_internal.h"
+
+#include "../../include/emsa_probe.h"
+
+namespace emsa {
+namespace {
+
+constexpr int kChains = 8;
+
+__global__ void __launch_bounds__(256) probe_exp_kernel(float* __restrict__ out, int iters) {
+    float a[kChains];
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) a[i] = 0.25f + 0.01f * (threadIdx.x & 7) + 0.05f * i;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < kChains; ++i) a[i] = __builtin_amdgcn_exp2f(-a[i]);  // stays in (0.5, 1)
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) s += a[i];
+    if (s == 12345.f) out[blockIdx.x] = s;  // keeps the chains live; never true
+}
+
+__global__ void __launch_bounds__(256) probe_fma_kernel(float* __restrict__ out, int iters, float b,
+                                                        float c) {
+    float a[kChains];
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) a[i] = 0.5f + 0.01f * (threadIdx.x & 7) + 0.05f * i;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < kChains; ++i) a[i] = fmaf(a[i], b, c);  // fixed point 1: bounded
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) s += a[i];
+    if (s == 12345.f) out[blockIdx.x] = s;
+}
+
+}  // namespace
+}  // namespace emsa
+
+using namespace emsa;
+
+extern "C" int emsa_probe_rates(double* exp_per_s, double* fma_per_s) {
+    clear_error();
+    if (!exp_per_s || !fma_per_s) return set_error("probe: null pointer"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc;
+    if ((rc = ctx->misc.reserve(sizeof(float) * 65536)) != EMSA_OK) return rc;
+    hipStream_t s = ctx->stream;
+    const int blocks = std::max(1, ctx->numCU) * 16, threads = 256, iters = 2048;
+    hipEvent_t e0, e1;
+    EMSA_HIP_TRY(hipEventCreate(&e0));
+    EMSA_HIP_TRY(hipEventCreate(&e1));
+    float ms = 0.f;
+    const double work = (double)blocks * threads * iters * kChains;
+    // warm-up, then timed
+    hipLaunchKernelGGL(probe_exp_kernel, dim3(blocks), dim3(threads), 0, s, ctx->misc.as<float>(),
+                       64);
+    EMSA_HIP_TRY(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(probe_exp_kernel, dim3(blocks), dim3(threads), 0, s, ctx->misc.as<float>(),
+                       iters);
+    EMSA_HIP_TRY(hipEventRecord(e1, s));
+    EMSA_HIP_TRY(hipEventSynchronize(e1));
+    EMSA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *exp_per_s = work / (ms * 1e-3);
+    hipLaunchKernelGGL(probe_fma_kernel, dim3(blocks), dim3(threads), 0, s, ctx->misc.as<float>(),
+                       64, 0.5f, 0.5f);
+    EMSA_HIP_TRY(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(probe_fma_kernel, dim3(blocks), dim3(threads), 0, s, ctx->misc.as<float>(),
+                       iters, 0.5f, 0.5f);
+    EMSA_HIP_TRY(hipEventRecord(e1, s));
+    EMSA_HIP_TRY(hipEventSynchronize(e1));
+    EMSA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *fma_per_s = work / (ms * 1e-3);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return EMSA_OK;
+}

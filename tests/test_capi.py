@@ -12,8 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_functions():
-    src = open(os.path.join(ROOT, "include", "emsa.h")).read()
-    return sorted(set(re.findall(r"\b(emsa_[a-z_0-9]+)\s*\(", src)))
+    import glob
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "emsa*.h")):
+        names |= set(re.findall(r"\b(emsa_[a-z_0-9]+)\s*\(", open(h).read()))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
