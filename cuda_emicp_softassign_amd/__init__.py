@@ -28,6 +28,8 @@ __all__ = [
     "registrationParameters", "emicp", "softassign", "findRTfromS", "cloud2data", "printRT",
     "emicp_traced", "emicp_step", "softassign_traced", "emicp_iterations", "emicp_dev",
     "EmsaError", "SOFTASSIGN_MATERIALISED", "SOFTASSIGN_LOGDOMAIN", "TRACE_W",
+    "emicp_batched", "emicp_batched_dev", "autotune_emicp", "multistart_rt0", "select_best",
+    "graph_cache_clear", "set_tiling",
 ]
 
 
@@ -216,3 +218,94 @@ def emicp_dev(X_soa, N, Y_soa, M, Rt, param=None, stream=0):
     return _lib.check("emicp_dev", _lib.load().emsa_emicp_dev(
         C.c_void_p(X_soa), N, C.c_void_p(Y_soa), M, C.c_void_p(Rt), C.byref(p),
         C.c_void_p(stream)))
+
+
+# ---------------------------------------------------------------------------
+# Batched EM-ICP (no counterpart in the reference)
+# ---------------------------------------------------------------------------
+
+def multistart_rt0(X, Y, rotations):
+    """Initial transforms for a multi-start run: each rotation R0 with the
+    translation that aligns the centroids, t0 = mean(X) - R0 mean(Y).
+    Returns (n_starts, 12) float32."""
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    out = []
+    for R0 in rotations:
+        R0 = np.asarray(R0, np.float64).reshape(3, 3)
+        out.append(np.concatenate([R0.reshape(9), X.mean(0) - R0 @ Y.mean(0)]))
+    return np.asarray(out, np.float32)
+
+
+def select_best(Rt, residual):
+    """Per pair, the (12,) result of the start with the lowest residual."""
+    Rt = np.asarray(Rt)
+    k = np.argmin(np.asarray(residual), axis=1)
+    return Rt[np.arange(Rt.shape[0]), k], k
+
+
+def emicp_batched(pairs, Rt0, param=None):
+    """Registers every (X, Y) in `pairs` from every initial transform in Rt0
+    ((count, n_starts, 12), or (n_starts, 12) for the same starts everywhere)
+    in one batched run. Returns (Rt (count, n_starts, 12), residual
+    (count, n_starts))."""
+    param = param or registrationParameters()
+    count = len(pairs)
+    Rt0 = np.asarray(Rt0, np.float32)
+    if Rt0.ndim == 2:
+        Rt0 = np.broadcast_to(Rt0, (count,) + Rt0.shape)
+    Rt0 = np.ascontiguousarray(Rt0)
+    n_starts = Rt0.shape[1]
+    keep = []
+    arr = (_lib.emsa_pair * count)()
+    for k, (X, Y) in enumerate(pairs):
+        xs, n = cloud2data(X)
+        ys, m = cloud2data(Y)
+        keep += [xs, ys]
+        arr[k] = _lib.emsa_pair(xs.ctypes.data, n, ys.ctypes.data, m)
+    out = np.zeros((count, n_starts, 12), np.float32)
+    res = np.zeros((count, n_starts))
+    p = param.to_c()
+    _lib.check("emicp_batched", _lib.load().emsa_emicp_batched(
+        arr, count, _fptr(Rt0), n_starts, C.byref(p), _fptr(out), _dptr(res)))
+    return out, res
+
+
+def _pairs_dev(dpairs):
+    arr = (_lib.emsa_pair * len(dpairs))()
+    for k, (x, n, y, m) in enumerate(dpairs):
+        arr[k] = _lib.emsa_pair(x, n, y, m)
+    return arr
+
+
+def emicp_batched_dev(dpairs, dRt0, n_starts, dRt_out, dResid=0, param=None, stream=0):
+    """Device-buffer batched EM-ICP. dpairs: list of (X_ptr, N, Y_ptr, M) with
+    device SoA pointers; dRt0/dRt_out: device [count*n_starts*12] float;
+    dResid: device [count*n_starts] double or 0. Replays a captured hipGraph
+    when called again with the same buffers."""
+    p = (param or registrationParameters()).to_c()
+    arr = _pairs_dev(dpairs)
+    return _lib.check("emicp_batched_dev", _lib.load().emsa_emicp_batched_dev(
+        arr, len(dpairs), C.c_void_p(dRt0), n_starts, C.byref(p), C.c_void_p(dRt_out),
+        C.c_void_p(dResid), C.c_void_p(stream)))
+
+
+def autotune_emicp(dpairs, n_starts=1, param=None, stream=0):
+    """Times the candidate decompositions on these device buffers and keeps
+    the fastest for later calls on this device; returns the choice."""
+    p = (param or registrationParameters()).to_c()
+    arr = _pairs_dev(dpairs)
+    choice = C.c_int(0)
+    _lib.check("autotune_emicp", _lib.load().emsa_autotune_emicp(
+        arr, len(dpairs), n_starts, C.byref(p), C.c_void_p(stream), C.byref(choice)))
+    return choice.value
+
+
+def set_tiling(blocks_per_cu=0, tile_cols=0):
+    """Manual override of the autotuner's choice on the current device
+    (0 restores the built-in heuristic)."""
+    return _lib.check("set_tiling", _lib.load().emsa_set_tiling(blocks_per_cu, tile_cols))
+
+
+def graph_cache_clear():
+    return _lib.check("graph_cache_clear", _lib.load().emsa_graph_cache_clear())

@@ -36,6 +36,11 @@ class emsa_params(C.Structure):
     ]
 
 
+class emsa_pair(C.Structure):
+    """Mirror of struct emsa_pair (include/emsa.h)."""
+    _fields_ = [("X", C.c_void_p), ("N", C.c_int), ("Y", C.c_void_p), ("M", C.c_int)]
+
+
 # Every function the header declares, with its ctypes signature.
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -55,6 +60,14 @@ SIGNATURES = {
     "emsa_emicp_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, _pp,
                                  C.c_void_p]),
     "emsa_probe_rates": (C.c_int, [_dp, _dp]),
+    "emsa_emicp_batched": (C.c_int, [C.POINTER(emsa_pair), C.c_int, _fp, C.c_int, _pp, _fp,
+                                     _dp]),
+    "emsa_emicp_batched_dev": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_void_p, C.c_int,
+                                         _pp, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "emsa_autotune_emicp": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_int, _pp, C.c_void_p,
+                                      C.POINTER(C.c_int)]),
+    "emsa_graph_cache_clear": (C.c_int, []),
+    "emsa_set_tiling": (C.c_int, [C.c_int, C.c_int]),
 }
 
 _lib = None

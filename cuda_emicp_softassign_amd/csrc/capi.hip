@@ -12,7 +12,10 @@ namespace emsa {
 namespace {
 thread_local std::string g_err;
 std::mutex g_ctx_mu;
-std::vector<std::unique_ptr<DeviceCtx>> g_ctx;
+// never destroyed: contexts outlive every static destructor that may still call in
+std::vector<std::unique_ptr<DeviceCtx>>& g_ctx = *new std::vector<std::unique_ptr<DeviceCtx>>();
+
+}  // namespace
 
 bool finite_params(const emsa_params& p) {
     const float f[] = {p.alpha, p.T_0, p.TFACTOR, p.moutlier, p.sigma_p2, p.sigma_inf,
@@ -27,6 +30,8 @@ bool finite_cloud(const float* P, int n) {
         if (!std::isfinite(P[i])) return false;
     return true;
 }
+
+namespace {
 
 __global__ void find_rt_kernel(const float* __restrict__ in, float* __restrict__ out) {
     if (threadIdx.x != 0) return;
@@ -46,6 +51,12 @@ __global__ void find_rt_kernel(const float* __restrict__ in, float* __restrict__
 
 void set_error(const std::string& msg) { g_err = msg; }
 void clear_error() { g_err.clear(); }
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
 
 int DevBuf::reserve(size_t want) {
     if (want <= bytes) return EMSA_OK;
@@ -315,21 +326,6 @@ int emsa_emicp_step(const float* X, int N, const float* Y, int M, float* R, floa
     const int rc = emicp_host(X, N, Y, M, R, t, p, sig, rec, 1, C, lambda);
     if (rc == EMSA_OK && record) std::memcpy(record, rec, sizeof(rec));
     return rc;
-}
-
-int emsa_emicp_dev(const float* dX, int N, const float* dY, int M, float* dRt,
-                   const emsa_params* p, void* stream) {
-    clear_error();
-    if (!dX || !dY || !dRt || !p || N <= 0 || M <= 0)
-        return set_error("emicp_dev: bad arguments"), EMSA_EINVAL;
-    if (!finite_params(*p)) return set_error("emicp_dev: non-finite params"), EMSA_ENONFINITE;
-    const std::vector<float> sig = sigma_schedule(*p);
-    if (sig.empty()) return EMSA_OK;
-    DeviceCtx* ctx = current_ctx();
-    if (!ctx) return EMSA_ENODEV;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    return emicp_device(*ctx, dX, N, dY, M, dRt, dRt, *p, sig, (hipStream_t)stream, nullptr,
-                        nullptr);
 }
 
 int emsa_softassign(const float* X, int N, const float* Y, int M, float* R, float* t,

@@ -1,4 +1,5 @@
-// EM-ICP on gfx950: one fused kernel launch per sigma^2 step.
+// EM-ICP on gfx950: one fused kernel launch per sigma^2 step, for one
+// registration or for a whole batch of them (uneven sizes, multi-start).
 //
 // Reference semantics (src/emicp.cu:443-668, src/emicp_cpu.cpp:305-527), with
 // rows r over the source Y (M) and columns c over the target X (N):
@@ -13,61 +14,80 @@
 //   C_r = sum e^2,  B_r = sum e,  W_r = sum e x_c
 // and then lambda_r = B_r / sqrt(C_r), lambda_r X'_r = W_r / sqrt(C_r).
 // Target points are wave-uniform, so they are read with scalar loads and
-// enter the VALU as SGPR operands; one exp2 and 12 VALU ops per pair.
+// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair.
 //
-// Work decomposition per launch: a workgroup = 64 source rows x one column
-// chunk; its 4 waves split the chunk. Partial sums go to HBM; the last
-// workgroup to finish a row block (atomic ticket) reduces the chunks in a
-// fixed order in FP64, forms the per-row normalisers and the 18 FP64 moments
-// of its rows; the last row block to finish sums the moments in a fixed order
-// and solves Horn's problem in FP64 on the device. No workgroup ever waits on
-// another, so the launch cannot deadlock, and the reduction order does not
-// depend on scheduling, so results are reproducible run to run.
+// Work decomposition per launch: a table of registrations, each cut into
+// tiles of 64 source rows x one column chunk; a workgroup is one tile and its
+// 4 waves split the chunk. Partial sums go to HBM; the last workgroup to
+// finish a row block (atomic ticket) reduces the chunks in a fixed order in
+// FP64, forms the per-row normalisers and the 18 FP64 moments of its rows; the
+// last row block of a registration to finish sums the moments in a fixed
+// order and solves Horn's problem in FP64 on the device. No workgroup ever
+// waits on another, so a launch cannot deadlock, and the reduction order does
+// not depend on scheduling, so results are reproducible run to run.
 #include "emsa_device.h"
 #include "emsa_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace emsa {
 
 namespace {
 
 constexpr float kSentinel = 1.0e15f;  // padding target: exp2 underflows to 0 for any sigma
+typedef const __attribute__((address_space(4))) float* cfloat_p;
 
-// Moments (FP64), per row block and in total:
-//  [0] Lambda            [1..3] sum lambda X'      [4..6] sum lambda y
-//  [7 + 3i + j] sum lambda X'_i y_j (i target axis, j source axis)
-//  [16] sum lambda |X' - (R y + t)|^2   [17] floored rows
-__global__ void __launch_bounds__(1024)
-emicp_setup_kernel(const float* __restrict__ X, int N, int ldx, float* __restrict__ Xpad,
-                   const float* __restrict__ Rt, double* __restrict__ state) {
+// Registration served by workgroup b: the last descriptor with tileBase <= b.
+__device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, int b) {
+    int lo = 0, hi = nregs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (regs[mid].tileBase <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One workgroup per registration: pad the target (once per target cloud),
+// sum it in FP64 (for floored rows) and load the initial transform.
+__global__ void __launch_bounds__(256)
+emicp_setup_kernel(const EmRegDesc* __restrict__ regs, const float* __restrict__ Rt0) {
+    const EmRegDesc& d = regs[blockIdx.x];
+    const int N = d.N, ldx = d.ldx;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int c = threadIdx.x; c < ldx; c += blockDim.x) {
         float x = kSentinel, y = kSentinel, z = kSentinel;
         if (c < N) {
-            x = X[c];
-            y = X[(size_t)N + c];
-            z = X[2 * (size_t)N + c];
+            x = d.X[c];
+            y = d.X[(size_t)N + c];
+            z = d.X[2 * (size_t)N + c];
             sx += x;
             sy += y;
             sz += z;
         }
-        Xpad[c] = x;
-        Xpad[(size_t)ldx + c] = y;
-        Xpad[2 * (size_t)ldx + c] = z;
+        if (d.padOwner) {
+            d.Xpad[c] = x;
+            d.Xpad[(size_t)ldx + c] = y;
+            d.Xpad[2 * (size_t)ldx + c] = z;
+        }
     }
-    __shared__ double lds[16 * 3];
+    __shared__ double lds[4 * 3];
     double v[3] = {sx, sy, sz};
     double tot;
     block_sum<3>(v, lds, blockDim.x / kWave, tot);
-    if (threadIdx.x < 3) state[kStXsum + threadIdx.x] = tot;
-    if (threadIdx.x < 12) state[threadIdx.x] = (double)Rt[threadIdx.x];
-    if (threadIdx.x == 12) state[kStStatus] = 0.0;
+    if (threadIdx.x < 3) d.state[kStXsum + threadIdx.x] = tot;
+    if (threadIdx.x < 12) d.state[threadIdx.x] = (double)Rt0[(size_t)blockIdx.x * 12 + threadIdx.x];
+    if (threadIdx.x == 12) d.state[kStStatus] = 0.0;
+    if (threadIdx.x == 13) d.state[kStResid] = 0.0;
 }
 
-__global__ void emicp_output_kernel(const double* __restrict__ state, float* __restrict__ Rt) {
-    if (threadIdx.x < 12) Rt[threadIdx.x] = (float)state[threadIdx.x];
+__global__ void emicp_output_kernel(const EmRegDesc* __restrict__ regs, float* __restrict__ Rt,
+                                    double* __restrict__ resid) {
+    const EmRegDesc& d = regs[blockIdx.x];
+    if (threadIdx.x < 12) Rt[(size_t)d.slot * 12 + threadIdx.x] = (float)d.state[threadIdx.x];
+    if (resid && threadIdx.x == 12) resid[d.slot] = d.state[kStResid];
 }
 
 // COMP: subtract the rounding residual of p_r in every difference,
@@ -75,21 +95,23 @@ __global__ void emicp_output_kernel(const double* __restrict__ state, float* __r
 // (Sterbenz), so d carries no error from rounding p_r to FP32.
 template <bool COMP>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, int N, int M,
-                  int ldx, int chunks, int waveCols, float negk, float sigma_p2, double outlier,
-                  double floorA, double* __restrict__ state, float* __restrict__ partial,
-                  double* __restrict__ moments, unsigned* __restrict__ counters,
-                  double* __restrict__ trace, double* __restrict__ rows, int iter) {
-    const int rowBlocks = gridDim.x / chunks;
-    const int rb = blockIdx.x / chunks;
-    const int g = blockIdx.x - rb * chunks;
+emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, float sigma_p2,
+                  double outlier, int iter, int last) {
+    const int ri = find_reg(regs, nregs, blockIdx.x);
+    const EmRegDesc& d = regs[ri];
+    const int N = d.N, M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
+    const int rowBlocks = d.rowBlocks;
+    double* __restrict__ state = d.state;
+    const int local = blockIdx.x - d.tileBase;
+    const int rb = local / chunks;
+    const int g = local - rb * chunks;
     const int lane = threadIdx.x & (kWave - 1);
     // wave index made provably uniform so that target loads become scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int r = rb * kEmRows + lane;
     const bool valid = r < M;
 
-    // p_r = R y_r + t in FP64, rounded once.
+    // p_r = R y_r + t in FP64, rounded once (residual kept for COMP).
     double Rm[9], tv[3];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Rm[i] = state[kStR + i];
@@ -97,9 +119,9 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
     for (int i = 0; i < 3; ++i) tv[i] = state[kStT + i];
     float yx = 0.f, yy = 0.f, yz = 0.f;
     if (valid) {
-        yx = Y[r];
-        yy = Y[(size_t)M + r];
-        yz = Y[2 * (size_t)M + r];
+        yx = d.Y[r];
+        yy = d.Y[(size_t)M + r];
+        yz = d.Y[2 * (size_t)M + r];
     }
     const double pxd = Rm[0] * yx + Rm[1] * yy + Rm[2] * yz + tv[0];
     const double pyd = Rm[3] * yx + Rm[4] * yy + Rm[5] * yz + tv[1];
@@ -108,9 +130,12 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
     const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
                 plz = (float)(pzd - (double)pz);
 
-    const float* __restrict__ Xx = Xpad;
-    const float* __restrict__ Xy = Xpad + ldx;
-    const float* __restrict__ Xz = Xpad + 2 * (size_t)ldx;
+    // The padded target is read-only during the launch: reading it through
+    // the constant address space with a wave-uniform index makes the loads
+    // scalar (s_load_dwordx16), and the values enter the VALU as SGPRs.
+    const cfloat_p Xx = (cfloat_p)d.Xpad;
+    const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
+    const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
     const int cb = (g * kEmWaves + wave) * waveCols;
     const int ce = min(cb + waveCols, ldx);
 
@@ -146,8 +171,10 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
     red[wave][4][lane] = Wz;
     __syncthreads();
     __shared__ unsigned s_ticket;
+    float* __restrict__ partial = d.partial;
+    unsigned* __restrict__ counters = d.counters;
     if (wave == 0) {
-        float* P = partial + (size_t)blockIdx.x * 5 * kEmRows;
+        float* P = partial + (size_t)local * 5 * kEmRows;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             float acc = red[0][q][lane];
@@ -163,6 +190,7 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
 
     // ---- last workgroup of row block rb: per-row normalisers and moments ----
     __shared__ double mom[kMomW];
+    double* __restrict__ moments = d.moments;
     if (wave == 0) {
         __threadfence();
         if (lane == 0) counters[rb] = 0u;
@@ -185,16 +213,16 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
             Vy = Ys * rs;
             Vz = Zs * rs;
         } else {  // emicp.cu:159: A' = 1/N for the whole row
-            lam = (double)N * floorA;
-            Vx = floorA * state[kStXsum + 0];
-            Vy = floorA * state[kStXsum + 1];
-            Vz = floorA * state[kStXsum + 2];
+            lam = (double)N * d.floorA;
+            Vx = d.floorA * state[kStXsum + 0];
+            Vy = d.floorA * state[kStXsum + 1];
+            Vz = d.floorA * state[kStXsum + 2];
             fl = 1.0;
         }
         if (!valid) lam = Vx = Vy = Vz = fl = 0.0;
-        if (rows && valid) {
-            rows[r] = Cs;
-            rows[(size_t)M + r] = lam;
+        if (last && d.rows && valid) {
+            d.rows[r] = Cs;
+            d.rows[(size_t)M + r] = lam;
         }
         double m[kMomW];
         m[0] = lam;
@@ -262,8 +290,8 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
         }
         state[kStResid] = mom[16] / L;
         state[kStLambda] = L;
-        if (trace) {
-            double* o = trace + (size_t)iter * EMSA_TRACE_W;
+        if (d.trace) {
+            double* o = d.trace + (size_t)iter * EMSA_TRACE_W;
             o[0] = sigma_p2;
             o[1] = L;
             for (int i = 0; i < 3; ++i) {
@@ -282,16 +310,13 @@ emicp_iter_kernel(const float* __restrict__ Xpad, const float* __restrict__ Y, i
 
 }  // namespace
 
-EmPlan plan_emicp(int N, int M, int numCU, int chunks_override) {
+EmPlan plan_emicp(int N, int M, int numCU, int blocks_per_cu) {
     EmPlan pl;
     pl.rowBlocks = (M + kEmRows - 1) / kEmRows;
     pl.ldx = (N + kColGroup - 1) / kColGroup * kColGroup;
     const int groups = pl.ldx / kColGroup;  // column groups of 16
-    int chunks = chunks_override;
-    if (chunks <= 0) {
-        const int target = std::max(1, numCU) * 8;  // workgroups to keep every CU busy
-        chunks = (target + pl.rowBlocks - 1) / pl.rowBlocks;
-    }
+    const int target = std::max(1, numCU) * (blocks_per_cu > 0 ? blocks_per_cu : 8);
+    int chunks = (target + pl.rowBlocks - 1) / pl.rowBlocks;
     chunks = std::max(1, std::min(chunks, (groups + kEmWaves - 1) / kEmWaves));
     const int gpw = (groups + chunks * kEmWaves - 1) / (chunks * kEmWaves);  // groups per wave
     pl.waveCols = gpw * kColGroup;
@@ -310,43 +335,114 @@ std::vector<float> sigma_schedule(const emsa_params& p) {
     return out;
 }
 
-int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
-                 const float* dRtIn, float* dRtOut, const emsa_params& p,
-                 const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows) {
-    const EmPlan pl = plan_emicp(N, M, ctx.numCU);
-    const size_t nblocks = (size_t)pl.rowBlocks * pl.chunks;
+int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
+               int tileCols, double* trace, double* rows, hipStream_t s) {
+    if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
+    w.regs.clear();
+    size_t xpadF = 0, partF = 0, momD = 0, cntU = 0;
+    long long tiles = 0;
+    std::vector<size_t> xoff(count);
+    for (int k = 0; k < count; ++k) {
+        const EmPair& pr = pairs[k];
+        if (!pr.X || !pr.Y || pr.N <= 0 || pr.M <= 0)
+            return set_error("emicp: null or empty cloud in batch"), EMSA_EINVAL;
+        EmPlan pl;
+        if (tileCols > 0) {
+            pl.rowBlocks = (pr.M + kEmRows - 1) / kEmRows;
+            pl.ldx = (pr.N + kColGroup - 1) / kColGroup * kColGroup;
+            pl.waveCols = std::max(kColGroup, tileCols / kEmWaves / kColGroup * kColGroup);
+            pl.chunks = (pl.ldx + pl.waveCols * kEmWaves - 1) / (pl.waveCols * kEmWaves);
+        } else {
+            pl = plan_emicp(pr.N, pr.M, ctx.numCU, ctx.emBlocksPerCU);
+        }
+        xoff[k] = xpadF;
+        xpadF += 3 * (size_t)pl.ldx;
+        for (int st = 0; st < n_starts; ++st) {
+            EmRegDesc d{};
+            d.X = pr.X;
+            d.Y = pr.Y;
+            d.N = pr.N;
+            d.M = pr.M;
+            d.ldx = pl.ldx;
+            d.rowBlocks = pl.rowBlocks;
+            d.chunks = pl.chunks;
+            d.waveCols = pl.waveCols;
+            d.floorA = (double)(1.0f / (float)pr.N);
+            d.padOwner = (st == 0);
+            d.slot = k * n_starts + st;
+            d.tileBase = (int)tiles;
+            const long long t = (long long)pl.rowBlocks * pl.chunks;
+            tiles += t;
+            // offsets for now, pointers below
+            d.partial = reinterpret_cast<float*>(partF);
+            partF += (size_t)t * 5 * kEmRows;
+            d.moments = reinterpret_cast<double*>(momD);
+            momD += (size_t)pl.rowBlocks * kMomW;
+            d.counters = reinterpret_cast<unsigned*>(cntU);
+            cntU += (size_t)pl.rowBlocks + 1;
+            d.trace = (count == 1 && n_starts == 1) ? trace : nullptr;
+            d.rows = (count == 1 && n_starts == 1) ? rows : nullptr;
+            w.regs.push_back(d);
+        }
+    }
+    if (tiles > 0x7fffffffLL) return set_error("emicp: batch too large for one launch"), EMSA_EINVAL;
+    w.tiles = tiles;
+    const int nregs = (int)w.regs.size();
     int rc;
-    if ((rc = ctx.Xpad.reserve(sizeof(float) * 3 * (size_t)pl.ldx)) != EMSA_OK) return rc;
-    if ((rc = ctx.partial.reserve(sizeof(float) * 5 * kEmRows * nblocks)) != EMSA_OK) return rc;
-    if ((rc = ctx.moments.reserve(sizeof(double) * kMomW * (size_t)pl.rowBlocks)) != EMSA_OK)
-        return rc;
-    const size_t oldc = ctx.counters.bytes;
-    if ((rc = ctx.counters.reserve(sizeof(unsigned) * ((size_t)pl.rowBlocks + 1))) != EMSA_OK)
-        return rc;
-    if (ctx.counters.bytes != oldc)
-        EMSA_HIP_TRY(hipMemsetAsync(ctx.counters.ptr, 0, ctx.counters.bytes, s));
-    if ((rc = ctx.state.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
+    if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
+    if ((rc = w.partial.reserve(sizeof(float) * partF)) != EMSA_OK) return rc;
+    if ((rc = w.moments.reserve(sizeof(double) * momD)) != EMSA_OK) return rc;
+    const size_t oldc = w.counters.bytes;
+    const void* oldp = w.counters.ptr;
+    if ((rc = w.counters.reserve(sizeof(unsigned) * cntU)) != EMSA_OK) return rc;
+    if (w.counters.bytes != oldc || w.counters.ptr != oldp)  // fresh tickets start at zero
+        EMSA_HIP_TRY(hipMemsetAsync(w.counters.ptr, 0, w.counters.bytes, s));
+    if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
+    if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
+    for (int i = 0; i < nregs; ++i) {
+        EmRegDesc& d = w.regs[i];
+        const int k = d.slot / n_starts;
+        d.Xpad = w.xpad.as<float>() + xoff[k];
+        d.partial = w.partial.as<float>() + reinterpret_cast<size_t>(d.partial);
+        d.moments = w.moments.as<double>() + reinterpret_cast<size_t>(d.moments);
+        d.counters = w.counters.as<unsigned>() + reinterpret_cast<size_t>(d.counters);
+        d.state = w.state.as<double>() + (size_t)i * kStateW;
+    }
+    EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(EmRegDesc) * nregs,
+                                hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));  // w.regs may be rewritten by the next prepare
+    return EMSA_OK;
+}
 
-    double* state = ctx.state.as<double>();
-    hipLaunchKernelGGL(emicp_setup_kernel, dim3(1), dim3(1024), 0, s, dX, N, pl.ldx,
-                       ctx.Xpad.as<float>(), dRtIn, state);
+int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
+               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s) {
+    (void)ctx;
+    const int nregs = (int)w.regs.size();
+    const EmRegDesc* desc = w.desc.as<EmRegDesc>();
+    hipLaunchKernelGGL(emicp_setup_kernel, dim3(nregs), dim3(256), 0, s, desc, dRt0);
     EMSA_HIP_TRY(hipGetLastError());
-    const double floorA = (double)(1.0f / (float)N);
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
-        const bool last = (it + 1 == sigmas.size());
-        hipLaunchKernelGGL(emicp_iter_kernel<true>, dim3((unsigned)nblocks),
-                           dim3(kEmRows * kEmWaves), 0, s, ctx.Xpad.as<float>(), dY, N, M, pl.ldx, pl.chunks, pl.waveCols,
-                           negk, sig, outlier, floorA, state, ctx.partial.as<float>(),
-                           ctx.moments.as<double>(), ctx.counters.as<unsigned>(), trace,
-                           last ? rows : nullptr, (int)it);
+        const int last = (it + 1 == sigmas.size()) ? 1 : 0;
+        hipLaunchKernelGGL(emicp_iter_kernel<true>, dim3((unsigned)w.tiles),
+                           dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk, sig, outlier,
+                           (int)it, last);
         EMSA_HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(emicp_output_kernel, dim3(1), dim3(64), 0, s, state, dRtOut);
+    hipLaunchKernelGGL(emicp_output_kernel, dim3(nregs), dim3(64), 0, s, desc, dRtOut, dResid);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
+}
+
+int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
+                 const float* dRtIn, float* dRtOut, const emsa_params& p,
+                 const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows) {
+    EmPair pr{dX, N, dY, M};
+    int rc = em_prepare(ctx, ctx.em, &pr, 1, 1, 0, trace, rows, s);
+    if (rc != EMSA_OK) return rc;
+    return em_enqueue(ctx, ctx.em, dRtIn, dRtOut, nullptr, p, sigmas, s);
 }
 
 }  // namespace emsa

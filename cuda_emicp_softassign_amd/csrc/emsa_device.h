@@ -41,12 +41,160 @@ __device__ inline void block_sum(double (&v)[K], double* lds, int nwaves, double
 }
 
 // ---------------------------------------------------------------------------
-// Horn's absolute orientation in FP64 (the maths of src/findRTfromS.cpp:33-159,
-// eigensolver: cyclic Jacobi in place of LAPACK dsyev_).
+// Horn's absolute orientation in FP64 (the maths of src/findRTfromS.cpp:33-159).
 // S[3a+b] = sum (source)_a (target)_b. Writes R (row-major) and t = Xc - R Yc.
-// Returns false when the input is not finite (R, t untouched).
+// Returns false when the input is not finite (R, t then unspecified).
+//
+// The top eigenvector of Horn's symmetric 4x4 matrix N replaces LAPACK dsyev_
+// (findRTfromS.cpp:118-155). Fast path, after the quaternion characteristic
+// polynomial method (QCP: Theobald 2005; Liu, Agrafiotis & Theobald 2010):
+// the largest root of det(N - l I) by Newton's method from a Gershgorin upper
+// bound (monotone, since all roots are real), then the eigenvector as the
+// largest column of adj(N - l I). When that column is tiny or the residual
+// check fails (near-degenerate top eigenvalue), a cyclic Jacobi solve is used.
 // ---------------------------------------------------------------------------
-__device__ inline bool horn_rt(const double* Xc, const double* Yc, const double* S, double* R,
+__host__ __device__ inline double det3(double a, double b, double c, double d, double e, double f,
+                              double g, double h, double i) {
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+__host__ __device__ inline void jacobi_top4(double A[4][4], double q[4]) {
+    double V[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double off = 0.0, diag = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            diag += A[i][i] * A[i][i];
+#pragma unroll
+            for (int j = i + 1; j < 4; ++j) off += A[i][j] * A[i][j];
+        }
+        if (off <= 1e-36 * diag || off == 0.0) break;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+#pragma unroll
+            for (int r = p + 1; r < 4; ++r) {
+                const double apq = A[p][r];
+                if (apq == 0.0) continue;
+                const double tau = (A[r][r] - A[p][p]) / (2.0 * apq);
+                const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + tt * tt);
+                const double s = tt * c;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double akp = A[k][p], akq = A[k][r];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][r] = s * akp + c * akq;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double apk = A[p][k], aqk = A[r][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[r][k] = s * apk + c * aqk;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][r];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][r] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    // eigenvector of the largest eigenvalue, selected without dynamic indexing
+    double best = A[0][0];
+    q[0] = V[0][0]; q[1] = V[1][0]; q[2] = V[2][0]; q[3] = V[3][0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        if (A[k][k] > best) {
+            best = A[k][k];
+            q[0] = V[0][k]; q[1] = V[1][k]; q[2] = V[2][k]; q[3] = V[3][k];
+        }
+    }
+}
+
+// Fast path; false when the caller must fall back to Jacobi.
+__host__ __device__ inline bool newton_top4(const double A[4][4], double q[4]) {
+    // Faddeev-LeVerrier: det(lI - A) = l^4 + c3 l^3 + c2 l^2 + c1 l + c0
+    double Mk[4][4], T[4][4];
+    const double c3 = -(A[0][0] + A[1][1] + A[2][2] + A[3][3]);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Mk[i][j] = A[i][j] + (i == j ? c3 : 0.0);
+    double cs[3];
+#pragma unroll
+    for (int k = 2; k <= 4; ++k) {
+        double tr = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double acc = 0.0;
+                for (int l = 0; l < 4; ++l) acc += A[i][l] * Mk[l][j];
+                T[i][j] = acc;
+            }
+        for (int i = 0; i < 4; ++i) tr += T[i][i];
+        const double c = -tr / k;
+        cs[k - 2] = c;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) Mk[i][j] = T[i][j] + (i == j ? c : 0.0);
+    }
+    const double c2 = cs[0], c1 = cs[1], c0 = cs[2];
+    double scale = 0.0, lam = -1e300;
+    for (int i = 0; i < 4; ++i) {
+        double r = A[i][i];
+        for (int j = 0; j < 4; ++j) {
+            scale = fmax(scale, fabs(A[i][j]));
+            if (j != i) r += fabs(A[i][j]);
+        }
+        lam = fmax(lam, r);  // Gershgorin: an upper bound of the largest eigenvalue
+    }
+    if (!(scale > 0.0)) return false;
+    for (int it = 0; it < 80; ++it) {
+        const double p = (((lam + c3) * lam + c2) * lam + c1) * lam + c0;
+        const double dp = ((4.0 * lam + 3.0 * c3) * lam + 2.0 * c2) * lam + c1;
+        if (!(dp > 0.0)) break;
+        const double step = p / dp;
+        lam -= step;
+        if (fabs(step) <= 1e-15 * scale) break;
+    }
+    // adjugate of B = A - lam I: row i of the cofactor matrix (B symmetric)
+    double B[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) B[i][j] = A[i][j] - (i == j ? lam : 0.0);
+    double best = -1.0;
+    for (int i = 0; i < 4; ++i) {
+        double v[4];
+        int rows[3], n = 0;
+        for (int k = 0; k < 4; ++k)
+            if (k != i) rows[n++] = k;
+        for (int j = 0; j < 4; ++j) {
+            int cols[3], m = 0;
+            for (int k = 0; k < 4; ++k)
+                if (k != j) cols[m++] = k;
+            const double dd = det3(B[rows[0]][cols[0]], B[rows[0]][cols[1]], B[rows[0]][cols[2]],
+                                   B[rows[1]][cols[0]], B[rows[1]][cols[1]], B[rows[1]][cols[2]],
+                                   B[rows[2]][cols[0]], B[rows[2]][cols[1]], B[rows[2]][cols[2]]);
+            v[j] = ((i + j) & 1) ? -dd : dd;
+        }
+        const double nn = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+        if (nn > best) {
+            best = nn;
+            for (int k = 0; k < 4; ++k) q[k] = v[k];
+        }
+    }
+    if (!(best > 1e-60 * scale * scale * scale * scale * scale * scale)) return false;
+    const double inv = 1.0 / sqrt(best);
+    for (int k = 0; k < 4; ++k) q[k] *= inv;
+    // residual check |A q - lam q| small against the matrix scale
+    double res = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double r = -lam * q[i];
+        for (int j = 0; j < 4; ++j) r += A[i][j] * q[j];
+        res = fmax(res, fabs(r));
+    }
+    return res <= 1e-10 * scale;
+}
+
+__host__ __device__ inline bool horn_rt(const double* Xc, const double* Yc, const double* S, double* R,
                                double* t) {
     double A[4][4];
     A[0][0] = S[0] + S[4] + S[8];
@@ -61,71 +209,14 @@ __device__ inline bool horn_rt(const double* Xc, const double* Yc, const double*
     A[3][3] = S[8] - S[0] - S[4];
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < i; ++j) A[i][j] = A[j][i];
-    double V[4][4];
+    double fro = 0.0;
     for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
-    double diag = 0.0, off = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        diag += A[i][i] * A[i][i];
-#pragma unroll
-        for (int j = i + 1; j < 4; ++j) off += A[i][j] * A[i][j];
-    }
-    if (!isfinite(diag) || !isfinite(off)) return false;
-    for (int sweep = 0; sweep < 32; ++sweep) {
-        off = 0.0;
-        diag = 0.0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            diag += A[i][i] * A[i][i];
-#pragma unroll
-            for (int j = i + 1; j < 4; ++j) off += A[i][j] * A[i][j];
-        }
-        if (off <= 1e-36 * diag || off == 0.0) break;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-#pragma unroll
-            for (int q = p + 1; q < 4; ++q) {
-                const double apq = A[p][q];
-                if (apq == 0.0) continue;
-                const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
-                const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double c = rsqrt(1.0 + tt * tt);
-                const double s = tt * c;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double akp = A[k][p], akq = A[k][q];
-                    A[k][p] = c * akp - s * akq;
-                    A[k][q] = s * akp + c * akq;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double apk = A[p][k], aqk = A[q][k];
-                    A[p][k] = c * apk - s * aqk;
-                    A[q][k] = s * apk + c * aqk;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double vkp = V[k][p], vkq = V[k][q];
-                    V[k][p] = c * vkp - s * vkq;
-                    V[k][q] = s * vkp + c * vkq;
-                }
-            }
-        }
-    }
-    // eigenvector of the largest eigenvalue, selected without dynamic indexing
-    double best = A[0][0], q0 = V[0][0], qx = V[1][0], qy = V[2][0], qz = V[3][0];
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-        if (A[k][k] > best) {
-            best = A[k][k];
-            q0 = V[0][k];
-            qx = V[1][k];
-            qy = V[2][k];
-            qz = V[3][k];
-        }
-    }
-    const double nrm = rsqrt(q0 * q0 + qx * qx + qy * qy + qz * qz);
+        for (int j = 0; j < 4; ++j) fro += A[i][j] * A[i][j];
+    if (!__builtin_isfinite(fro)) return false;
+    double q[4];
+    if (!newton_top4(A, q)) jacobi_top4(A, q);
+    double q0 = q[0], qx = q[1], qy = q[2], qz = q[3];
+    const double nrm = 1.0 / sqrt(q0 * q0 + qx * qx + qy * qy + qz * qz);
     q0 *= nrm; qx *= nrm; qy *= nrm; qz *= nrm;
     R[0] = q0 * q0 + qx * qx - qy * qy - qz * qz;
     R[1] = 2.0 * (qx * qy - q0 * qz);
@@ -139,7 +230,7 @@ __device__ inline bool horn_rt(const double* Xc, const double* Yc, const double*
     for (int i = 0; i < 3; ++i)
         t[i] = Xc[i] - (R[3 * i] * Yc[0] + R[3 * i + 1] * Yc[1] + R[3 * i + 2] * Yc[2]);
     for (int i = 0; i < 9; ++i)
-        if (!isfinite(R[i])) return false;
+        if (!__builtin_isfinite(R[i])) return false;
     return true;
 }
 

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -14,6 +15,8 @@ namespace emsa {
 
 void set_error(const std::string& msg);
 void clear_error();
+bool finite_params(const emsa_params& p);
+bool finite_cloud(const float* P, int n);
 
 #define EMSA_HIP_TRY(call)                                                               \
     do {                                                                                 \
@@ -24,11 +27,17 @@ void clear_error();
         }                                                                                \
     } while (0)
 
-// Grow-only device buffer.
+// Grow-only device buffer. Not freed by a destructor: contexts live until the
+// process ends (freeing after the runtime's own teardown is unsafe); graph
+// cache entries call release() when they are evicted.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     int reserve(size_t want);  // EMSA_OK / EMSA_ENOMEM
+    void release();
     template <class T>
     T* as() const { return static_cast<T*>(ptr); }
 };
@@ -48,15 +57,67 @@ struct EmPlan {
     int ldx = 0;         // padded column count = roundup(N, kColGroup)
 };
 
+// Device-side descriptor of one EM-ICP registration (one source/target pair
+// and one initial transform). A launch covers a table of them; workgroup b
+// serves the registration whose [tileBase, tileBase + rowBlocks*chunks)
+// contains b.
+struct EmRegDesc {
+    const float* X;      // target, SoA [3][N] as given
+    float* Xpad;         // target, SoA [3][ldx], padded with far sentinels
+    const float* Y;      // source, SoA [3][M]
+    double* state;       // kStateW doubles
+    float* partial;      // rowBlocks*chunks*5*kEmRows chunk partial sums
+    double* moments;     // rowBlocks*kMomW row-block moments
+    unsigned* counters;  // rowBlocks + 1 arrival tickets
+    double* trace;       // nullable: EMSA_TRACE_W doubles per sigma step
+    double* rows;        // nullable: C_r, lambda_r (2*M) of the last step
+    double floorA;       // (double)(1.0f / N), the floored affinity
+    int N, M, ldx, rowBlocks, chunks, waveCols;
+    int tileBase;        // first workgroup index of this registration
+    int padOwner;        // 1: this registration writes Xpad (once per target cloud)
+    int slot;            // output slot
+    int pad_;
+};
+
+// One source/target pair of an EM-ICP batch (device or host pointers,
+// depending on the entry point).
+struct EmPair {
+    const float* X;
+    int N;
+    const float* Y;
+    int M;
+};
+
+// Workspace and descriptor table of one EM-ICP batch.
+struct EmWork {
+    DevBuf xpad, partial, moments, counters, state, desc;
+    std::vector<EmRegDesc> regs;
+    long long tiles = 0;
+};
+
+struct GraphEntry {
+    std::vector<uint64_t> key;
+    EmWork work;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t lastUse = 0;
+};
+
 struct DeviceCtx {
     int device = 0;
     int numCU = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    // workspaces
-    DevBuf X, Y, Xpad, partial, moments, counters, state, rt, trace, rows, misc;
+    // workspaces of the eager (non-graph) entry points
+    DevBuf X, Y, rt, trace, rows, misc, resid;
+    EmWork em;
     // softassign workspaces
-    DevBuf K, u, v, orow, ocol, rowsum, colsum, colpart, rowpart, sacnt, samom;
+    DevBuf K, u, v, orow, ocol, rowsum, colsum, colpart, rowpart, sacnt, samom, sastate;
+    // captured EM-ICP launch sequences of the device entry points, replayed
+    // when a call repeats the same buffers, sizes and parameters
+    std::vector<std::unique_ptr<GraphEntry>> graphs;
+    int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
+    int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)
 };
 
 // Returns the context of the current HIP device (created on first use) or
@@ -68,10 +129,19 @@ std::vector<float> sigma_schedule(const emsa_params& p);
 
 EmPlan plan_emicp(int N, int M, int numCU, int chunks_override = 0);
 
-// Runs the EM-ICP loop on device buffers. dX, dY: SoA device arrays.
-// dRtIn/dRtOut: 12 floats (R row-major, t). trace: device array of
-// n*EMSA_TRACE_W doubles or null. rows: device array of 2*M doubles (C, lambda
-// of the last step) or null.
+// Lays out a batch of count pairs x n_starts initial transforms in w
+// (allocating its buffers and uploading the descriptors on stream s).
+// tileCols > 0: fixed column tile per workgroup (batched); 0: the single-
+// registration plan that spreads one registration over the whole device.
+int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
+               int tileCols, double* trace, double* rows, hipStream_t s);
+
+// Enqueues setup, one launch per sigma^2 step, and the output kernel.
+// dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.
+int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
+               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s);
+
+// Runs the EM-ICP loop on device buffers for one registration (eager).
 int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                  const float* dRtIn, float* dRtOut, const emsa_params& p,
                  const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows);

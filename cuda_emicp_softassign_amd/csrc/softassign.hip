@@ -489,8 +489,8 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         EMSA_HIP_TRY(hipMemsetAsync(ctx.sacnt.ptr, 0, ctx.sacnt.bytes, s));
     b.rcnt = ctx.sacnt.as<unsigned>();
     b.ccnt = b.rcnt + pl.rowBlocks + 1;
-    if ((rc = ctx.state.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
-    b.state = ctx.state.as<double>();
+    if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
+    b.state = ctx.sastate.as<double>();
     b.trace = trace;
 
     hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,

@@ -116,6 +116,50 @@ int emsa_find_rt_from_s(const float *Xc, const float *Yc, const float *S, float 
 int emsa_emicp_dev(const float *dX, int N, const float *dY, int M, float *dRt,
                    const emsa_params *p, void *stream);
 
+/* ---- Batched EM-ICP (no counterpart in the reference) -------------------
+ * count source/target pairs of uneven sizes, each registered from n_starts
+ * initial transforms, all in one kernel launch per sigma^2 step (the same
+ * schedule for every registration). Rt0: [count][n_starts][12] initial
+ * (R row-major, t); Rt_out: [count][n_starts][12]; residual (nullable):
+ * [count][n_starts], sum_r lambda_r |X'_r - (R y_r + t)|^2 / Lambda of the
+ * last sigma^2 step (with the R, t that step started from). */
+typedef struct emsa_pair {
+    const float *X; /* target, SoA, N points */
+    int N;
+    const float *Y; /* source, SoA, M points */
+    int M;
+} emsa_pair;
+
+/* Host buffers (pairs point to host SoA arrays). */
+int emsa_emicp_batched(const emsa_pair *pairs, int count, const float *Rt0, int n_starts,
+                       const emsa_params *p, float *Rt_out, double *residual);
+
+/* Device buffers (pairs point to device SoA arrays; dRt0, dRt_out, dResidual
+ * are device arrays). The launch sequence is captured once in a hipGraph and
+ * replayed on `stream` whenever a call repeats the same buffers, sizes and
+ * parameters. Asynchronous; errors detected at launch are returned. */
+int emsa_emicp_batched_dev(const emsa_pair *pairs, int count, const float *dRt0, int n_starts,
+                           const emsa_params *p, float *dRt_out, double *dResidual,
+                           void *stream);
+
+/* Autotuner: times the candidate work decompositions on these device buffers
+ * (the first sigma^2 steps of p) and keeps the fastest for later calls on the
+ * current device. count == 1 && n_starts == 1 tunes the single-registration
+ * plan (workgroups per CU), otherwise the batched column tile width. The
+ * choice is written to *choice (nullable). Results of different choices
+ * differ only by FP32 summation order. */
+int emsa_autotune_emicp(const emsa_pair *pairs, int count, int n_starts, const emsa_params *p,
+                        void *stream, int *choice);
+
+/* Manual override of the decomposition the autotuner chooses, for the
+ * current device: workgroups per CU of the single-registration plan and the
+ * batched column tile width (0 keeps the built-in heuristic). */
+int emsa_set_tiling(int blocks_per_cu, int tile_cols);
+
+/* Drops every captured launch sequence of the current device and its
+ * workspace. */
+int emsa_graph_cache_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

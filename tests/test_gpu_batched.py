@@ -1,0 +1,129 @@
+"""Batched multi-start EM-ICP, its hipGraph replay and the autotuner.
+
+Bar: every registration of a batch equals the plain single-registration run
+of the same pair and start to 1e-6 rad / 1e-6 (only the FP32 summation order
+of the partial sums differs); graph replays are bit-identical to the eager
+batched run; the multi-start selection finds the generating rotation.
+"""
+import numpy as np
+import pytest
+
+from cuda_emicp_softassign_amd.clouds import (geodesic, octahedral_rotations, range_scans,
+                                              rotation, uniform_cube)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def E():
+    import cuda_emicp_softassign_amd as E
+    E._lib.load()
+    return E
+
+
+def _pairs():
+    out = []
+    for k, (n, m) in enumerate([(60, 1500), (2500, 700), (1007, 1007), (33, 17), (4100, 3900)]):
+        X, Y, R, t = range_scans(n, m, seed=100 + k)
+        out.append((X, Y))
+    return out
+
+
+def test_batched_matches_plain(E):
+    pairs = _pairs()
+    starts = [np.eye(3), rotation([0, 0, 1], 0.3), rotation([1, 0, 0], -0.2)]
+    Rt0 = np.stack([E.multistart_rt0(X, Y, starts) for X, Y in pairs])
+    p = E.registrationParameters(sigma_inf=2e-4)
+    Rt, res = E.emicp_batched(pairs, Rt0, p)
+    assert Rt.shape == (len(pairs), 3, 12) and res.shape == (len(pairs), 3)
+    assert np.isfinite(res).all() and (res >= 0).all()
+    for k, (X, Y) in enumerate(pairs):
+        for s in range(3):
+            R, t = E.emicp(X, Y, Rt0[k, s, :9].reshape(3, 3), Rt0[k, s, 9:], p)
+            assert geodesic(Rt[k, s, :9].reshape(3, 3).astype(float), R.astype(float)) < 1e-6
+            np.testing.assert_allclose(Rt[k, s, 9:], t, atol=1e-6)
+
+
+def test_multistart_finds_large_rotation(E):
+    """A 150 degree offset is outside EM-ICP's basin from R = I; the best of
+    the 24 octahedral starts recovers it."""
+    X, Y, _, _ = uniform_cube(800, seed=4, max_deg=0.0)
+    Rg = rotation([1, 2, 2], np.deg2rad(150))
+    Y = (Y @ Rg).astype(np.float32)  # X ~= Rg Y
+    Rt0 = E.multistart_rt0(X, Y, octahedral_rotations())
+    Rt, res = E.emicp_batched([(X, Y)], Rt0)
+    best, k = E.select_best(Rt, res)
+    assert geodesic(best[0, :9].reshape(3, 3).astype(float), Rg) < 2e-3
+    R1, _ = E.emicp(X, Y, np.eye(3), np.zeros(3))
+    assert geodesic(R1.astype(float), Rg) > 0.1  # the single start does not get there
+
+
+def test_graph_replays_match_eager(E):
+    import torch
+    pairs = _pairs()
+    starts = [np.eye(3), rotation([0, 1, 0], 0.25)]
+    Rt0 = np.stack([E.multistart_rt0(X, Y, starts) for X, Y in pairs]).astype(np.float32)
+    p = E.registrationParameters(sigma_inf=2e-4)
+    ref, ref_res = E.emicp_batched(pairs, Rt0, p)
+    keep = []
+    dpairs = []
+    for X, Y in pairs:
+        dx = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+        dy = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()
+        keep += [dx, dy]
+        dpairs.append((dx.data_ptr(), len(X), dy.data_ptr(), len(Y)))
+    n = len(pairs) * 2
+    d0 = torch.from_numpy(Rt0.reshape(-1)).cuda()
+    dout = torch.zeros(12 * n, device="cuda")
+    dres = torch.zeros(n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    for rep in range(3):
+        dout.fill_(float("nan"))
+        E.emicp_batched_dev(dpairs, d0.data_ptr(), 2, dout.data_ptr(), dres.data_ptr(), p,
+                            stream)
+        torch.cuda.synchronize()
+        got = dout.cpu().numpy().reshape(len(pairs), 2, 12)
+        assert np.array_equal(got, ref), rep
+        assert np.array_equal(dres.cpu().numpy().reshape(len(pairs), 2), ref_res), rep
+        # eager work between replays
+        E.emicp(pairs[0][0], pairs[0][1], np.eye(3), np.zeros(3))
+        torch.cuda.synchronize()
+
+
+def test_single_registration_graph_replays(E):
+    import torch
+    X, Y, _, _ = uniform_cube(3000, seed=8)
+    R, t = E.emicp(X, Y, np.eye(3), np.zeros(3))
+    dX = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    dY = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()
+    rt0 = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float32, device="cuda")
+    dRt = rt0.clone()
+    s = torch.cuda.current_stream().cuda_stream
+    for rep in range(3):
+        dRt.copy_(rt0)
+        E.emicp_dev(dX.data_ptr(), 3000, dY.data_ptr(), 3000, dRt.data_ptr(), stream=s)
+        torch.cuda.synchronize()
+        out = dRt.cpu().numpy()
+        assert np.array_equal(out[:9], R.reshape(9)) and np.array_equal(out[9:], t), rep
+        dX.mul_(1.0)  # eager torch work in between
+        torch.cuda.synchronize()
+
+
+def test_autotune(E):
+    import torch
+    pairs = _pairs()
+    keep, dpairs = [], []
+    for X, Y in pairs:
+        dx = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+        dy = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()
+        keep += [dx, dy]
+        dpairs.append((dx.data_ptr(), len(X), dy.data_ptr(), len(Y)))
+    c = E.autotune_emicp(dpairs, 4)
+    assert c in (256, 512, 1024, 2048, 4096)
+    c1 = E.autotune_emicp(dpairs[:1], 1)
+    assert c1 in (2, 4, 8, 16, 32)
+    X, Y = pairs[2]
+    R, t = E.emicp(X, Y, np.eye(3), np.zeros(3))
+    Rt, res = E.emicp_batched(pairs[2:3], np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]]))
+    assert geodesic(Rt[0, 0, :9].reshape(3, 3).astype(float), R.astype(float)) < 1e-6
+    E.graph_cache_clear()
