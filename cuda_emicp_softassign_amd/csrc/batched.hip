@@ -12,7 +12,7 @@ namespace emsa {
 namespace {
 
 constexpr size_t kMaxGraphs = 4;
-constexpr int kDefaultTileCols = 1024;
+constexpr int kDefaultTileCols = 2048;
 uint64_t g_clock = 0;
 
 uint64_t fbits(float f) {
@@ -36,7 +36,6 @@ void drop_entry(GraphEntry& e) {
     e.work.xpad.release();
     e.work.partial.release();
     e.work.moments.release();
-    e.work.counters.release();
     e.work.state.release();
     e.work.desc.release();
 }
@@ -239,7 +238,7 @@ int emsa_autotune_emicp(const emsa_pair* pairs, int count, int n_starts, const e
     hipStream_t s = (hipStream_t)stream;
     const bool single = (count == 1 && n_starts == 1);
     const std::vector<int> cands = single ? std::vector<int>{2, 4, 8, 16, 32}
-                                          : std::vector<int>{256, 512, 1024, 2048, 4096};
+                                          : std::vector<int>{512, 1024, 2048, 4096, 8192};
     const size_t nregs = (size_t)count * n_starts;
     if ((rc = ctx->rt.reserve(sizeof(float) * 24 * nregs)) != EMSA_OK) return rc;
     std::vector<float> rt0(12 * nregs, 0.f);

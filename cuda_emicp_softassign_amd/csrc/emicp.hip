@@ -16,15 +16,20 @@
 // Target points are wave-uniform, so they are read with scalar loads and
 // enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair.
 //
-// Work decomposition per launch: a table of registrations, each cut into
-// tiles of 64 source rows x one column chunk; a workgroup is one tile and its
-// 4 waves split the chunk. Partial sums go to HBM; the last workgroup to
-// finish a row block (atomic ticket) reduces the chunks in a fixed order in
-// FP64, forms the per-row normalisers and the 18 FP64 moments of its rows; the
-// last row block of a registration to finish sums the moments in a fixed
-// order and solves Horn's problem in FP64 on the device. No workgroup ever
-// waits on another, so a launch cannot deadlock, and the reduction order does
-// not depend on scheduling, so results are reproducible run to run.
+// Work decomposition per sigma^2 step, three launches with no signalling
+// between workgroups (kernel boundaries order them):
+//  1. emicp_affinity_kernel: a table of registrations, each cut into tiles of
+//     64 source rows x one column chunk; a workgroup is one tile, its 4 waves
+//     split the chunk and combine in LDS; the tile's per-row partial sums
+//     (C, B, W) go to HBM.
+//  2. emicp_rows_kernel: one wave per row block reduces the chunk partials in
+//     a fixed order in FP64, forms lambda_r and lambda_r X'_r (or the floored
+//     row) and the 18 FP64 moments of its 64 rows.
+//  3. emicp_solve_kernel: one workgroup per registration sums the row-block
+//     moments in a fixed order, forms Xc, Yc, S and solves Horn's problem in
+//     FP64, updating (R, t).
+// The reduction order never depends on scheduling, so results are
+// reproducible run to run.
 #include "emsa_device.h"
 #include "emsa_internal.h"
 
@@ -90,18 +95,40 @@ __global__ void emicp_output_kernel(const EmRegDesc* __restrict__ regs, float* _
     if (resid && threadIdx.x == 12) resid[d.slot] = d.state[kStResid];
 }
 
-// COMP: subtract the rounding residual of p_r in every difference,
+// Registration owning global row block gr: the last descriptor with
+// rowBase <= gr.
+__device__ inline int find_reg_rows(const EmRegDesc* __restrict__ regs, int nregs, int gr) {
+    int lo = 0, hi = nregs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (regs[mid].rowBase <= gr) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// p_r = R y_r + t in FP64 from the registration state (rounded by the caller).
+__device__ inline void transform_row(const double* __restrict__ state, float yx, float yy,
+                                     float yz, double& pxd, double& pyd, double& pzd) {
+    double Rm[9], tv[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Rm[i] = state[kStR + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tv[i] = state[kStT + i];
+    pxd = Rm[0] * yx + Rm[1] * yy + Rm[2] * yz + tv[0];
+    pyd = Rm[3] * yx + Rm[4] * yy + Rm[5] * yz + tv[1];
+    pzd = Rm[6] * yx + Rm[7] * yy + Rm[8] * yz + tv[2];
+}
+
+// Stage 1. COMP: subtract the rounding residual of p_r in every difference,
 // d = (x - p_hi) - p_lo, where x - p_hi is exact for the pairs that matter
 // (Sterbenz), so d carries no error from rounding p_r to FP32.
 template <bool COMP>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, float sigma_p2,
-                  double outlier, int iter, int last) {
+emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk) {
     const int ri = find_reg(regs, nregs, blockIdx.x);
     const EmRegDesc& d = regs[ri];
-    const int N = d.N, M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
-    const int rowBlocks = d.rowBlocks;
-    double* __restrict__ state = d.state;
+    const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
     const int local = blockIdx.x - d.tileBase;
     const int rb = local / chunks;
     const int g = local - rb * chunks;
@@ -109,23 +136,15 @@ emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, flo
     // wave index made provably uniform so that target loads become scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int r = rb * kEmRows + lane;
-    const bool valid = r < M;
 
-    // p_r = R y_r + t in FP64, rounded once (residual kept for COMP).
-    double Rm[9], tv[3];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Rm[i] = state[kStR + i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) tv[i] = state[kStT + i];
     float yx = 0.f, yy = 0.f, yz = 0.f;
-    if (valid) {
+    if (r < M) {
         yx = d.Y[r];
         yy = d.Y[(size_t)M + r];
         yz = d.Y[2 * (size_t)M + r];
     }
-    const double pxd = Rm[0] * yx + Rm[1] * yy + Rm[2] * yz + tv[0];
-    const double pyd = Rm[3] * yx + Rm[4] * yy + Rm[5] * yz + tv[1];
-    const double pzd = Rm[6] * yx + Rm[7] * yy + Rm[8] * yz + tv[2];
+    double pxd, pyd, pzd;
+    transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
     const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
     const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
                 plz = (float)(pzd - (double)pz);
@@ -162,7 +181,7 @@ emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, flo
         }
     }
 
-    // Combine the 4 waves of the chunk in LDS, fixed order.
+    // Combine the 4 waves of the chunk in LDS, fixed order; wave 0 stores.
     __shared__ float red[kEmWaves][5][kEmRows];
     red[wave][0][lane] = C;
     red[wave][1][lane] = B;
@@ -170,11 +189,8 @@ emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, flo
     red[wave][3][lane] = Wy;
     red[wave][4][lane] = Wz;
     __syncthreads();
-    __shared__ unsigned s_ticket;
-    float* __restrict__ partial = d.partial;
-    unsigned* __restrict__ counters = d.counters;
     if (wave == 0) {
-        float* P = partial + (size_t)local * 5 * kEmRows;
+        float* P = d.partial + (size_t)local * 5 * kEmRows;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             float acc = red[0][q][lane];
@@ -182,129 +198,165 @@ emicp_iter_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, flo
             for (int w = 1; w < kEmWaves; ++w) acc += red[w][q][lane];
             P[q * kEmRows + lane] = acc;
         }
-        __threadfence();
-        if (lane == 0) s_ticket = atomicAdd(&counters[rb], 1u);
     }
-    __syncthreads();
-    if (s_ticket != (unsigned)(chunks - 1)) return;  // not the last chunk of this row block
+}
 
-    // ---- last workgroup of row block rb: per-row normalisers and moments ----
+// Stage 2: one wave per row block (64 source rows).
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
+                  double outlier, int last) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long gr = (long long)blockIdx.x * kEmWaves + wave;
+    if (gr >= totalRowBlocks) return;  // wave-uniform
+    const int lane = threadIdx.x & (kWave - 1);
+    const int ri = find_reg_rows(regs, nregs, (int)gr);
+    const EmRegDesc& d = regs[ri];
+    const int N = d.N, M = d.M, chunks = d.chunks;
+    const int rb = (int)gr - d.rowBase;
+    const int r = rb * kEmRows + lane;
+    const bool valid = r < M;
+    const double* __restrict__ state = d.state;
+
+    // chunk partials of this row block, summed in chunk order in FP64; the
+    // loads of 4 chunks are issued before their adds
+    double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
+    const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
+    int k = 0;
+    for (; k + 4 <= chunks; k += 4) {
+        float v[4][5];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v[i][q] = Pr[((size_t)(k + i) * 5 + q) * kEmRows];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            Cs += v[i][0];
+            Bs += v[i][1];
+            Xs += v[i][2];
+            Ys += v[i][3];
+            Zs += v[i][4];
+        }
+    }
+    for (; k < chunks; ++k) {
+        const float* q = Pr + (size_t)k * 5 * kEmRows;
+        Cs += q[0];
+        Bs += q[kEmRows];
+        Xs += q[2 * kEmRows];
+        Ys += q[3 * kEmRows];
+        Zs += q[4 * kEmRows];
+    }
+    Cs += outlier;
+    double lam, Vx, Vy, Vz, fl = 0.0;
+    if ((float)Cs > 10e-7f) {  // emicp.cu:155, compared in float
+        const double rs = 1.0 / sqrt(Cs);
+        lam = Bs * rs;
+        Vx = Xs * rs;
+        Vy = Ys * rs;
+        Vz = Zs * rs;
+    } else {  // emicp.cu:159: A' = 1/N for the whole row
+        lam = (double)N * d.floorA;
+        Vx = d.floorA * state[kStXsum + 0];
+        Vy = d.floorA * state[kStXsum + 1];
+        Vz = d.floorA * state[kStXsum + 2];
+        fl = 1.0;
+    }
+    float yx = 0.f, yy = 0.f, yz = 0.f;
+    if (valid) {
+        yx = d.Y[r];
+        yy = d.Y[(size_t)M + r];
+        yz = d.Y[2 * (size_t)M + r];
+    } else {
+        lam = Vx = Vy = Vz = fl = 0.0;
+    }
+    if (last && d.rows && valid) {
+        d.rows[r] = Cs;
+        d.rows[(size_t)M + r] = lam;
+    }
+    double pxd, pyd, pzd;
+    transform_row(state, yx, yy, yz, pxd, pyd, pzd);
+    double m[kMomW];
+    m[0] = lam;
+    m[1] = Vx;
+    m[2] = Vy;
+    m[3] = Vz;
+    m[4] = lam * yx;
+    m[5] = lam * yy;
+    m[6] = lam * yz;
+    const double V[3] = {Vx, Vy, Vz};
+    const double yv[3] = {(double)yx, (double)yy, (double)yz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) m[7 + 3 * i + j] = V[i] * yv[j];
+    double res = 0.0;
+    if (lam > 0.0) {
+        const double ex = Vx / lam - pxd, ey = Vy / lam - pyd, ez = Vz / lam - pzd;
+        res = lam * (ex * ex + ey * ey + ez * ez);
+    }
+    m[16] = res;
+    m[17] = fl;
+#pragma unroll
+    for (int q = 0; q < kMomW; ++q) m[q] = wave_sum(m[q]);
+    if (lane < kMomW) {
+        double v = m[0];
+#pragma unroll
+        for (int q = 1; q < kMomW; ++q)
+            if (lane == q) v = m[q];
+        d.moments[(size_t)lane * d.rowBlocks + rb] = v;
+    }
+}
+
+// Stage 3: one workgroup per registration.
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
+    const EmRegDesc& d = regs[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int rowBlocks = d.rowBlocks;
     __shared__ double mom[kMomW];
-    double* __restrict__ moments = d.moments;
-    if (wave == 0) {
-        __threadfence();
-        if (lane == 0) counters[rb] = 0u;
-        double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
-        const float* Pr = partial + (size_t)rb * chunks * 5 * kEmRows;
-        for (int k = 0; k < chunks; ++k) {
-            const float* q = Pr + (size_t)k * 5 * kEmRows;
-            Cs += q[lane];
-            Bs += q[kEmRows + lane];
-            Xs += q[2 * kEmRows + lane];
-            Ys += q[3 * kEmRows + lane];
-            Zs += q[4 * kEmRows + lane];
-        }
-        Cs += outlier;
-        double lam, Vx, Vy, Vz, fl = 0.0;
-        if ((float)Cs > 10e-7f) {  // emicp.cu:155, compared in float
-            const double rs = 1.0 / sqrt(Cs);
-            lam = Bs * rs;
-            Vx = Xs * rs;
-            Vy = Ys * rs;
-            Vz = Zs * rs;
-        } else {  // emicp.cu:159: A' = 1/N for the whole row
-            lam = (double)N * d.floorA;
-            Vx = d.floorA * state[kStXsum + 0];
-            Vy = d.floorA * state[kStXsum + 1];
-            Vz = d.floorA * state[kStXsum + 2];
-            fl = 1.0;
-        }
-        if (!valid) lam = Vx = Vy = Vz = fl = 0.0;
-        if (last && d.rows && valid) {
-            d.rows[r] = Cs;
-            d.rows[(size_t)M + r] = lam;
-        }
-        double m[kMomW];
-        m[0] = lam;
-        m[1] = Vx;
-        m[2] = Vy;
-        m[3] = Vz;
-        m[4] = lam * yx;
-        m[5] = lam * yy;
-        m[6] = lam * yz;
-        const double V[3] = {Vx, Vy, Vz};
-        const double yv[3] = {(double)yx, (double)yy, (double)yz};
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) m[7 + 3 * i + j] = V[i] * yv[j];
-        double res = 0.0;
-        if (lam > 0.0) {
-            const double ex = Vx / lam - pxd, ey = Vy / lam - pyd, ez = Vz / lam - pzd;
-            res = lam * (ex * ex + ey * ey + ez * ez);
-        }
-        m[16] = res;
-        m[17] = fl;
-#pragma unroll
-        for (int k = 0; k < kMomW; ++k) m[k] = wave_sum(m[k]);
-        if (lane < kMomW) {
-            double v = m[0];
-#pragma unroll
-            for (int k = 1; k < kMomW; ++k)
-                if (lane == k) v = m[k];
-            moments[(size_t)rb * kMomW + lane] = v;
-        }
-        __threadfence();
-        if (lane == 0) s_ticket = atomicAdd(&counters[rowBlocks], 1u);
+    // moment q: lanes stride over the row blocks, then a fixed-order wave sum
+    for (int q = wave; q < kMomW; q += kEmWaves) {
+        const double* src = d.moments + (size_t)q * rowBlocks;
+        double s = 0.0;
+        for (int b = lane; b < rowBlocks; b += kWave) s += src[b];
+        s = wave_sum(s);
+        if (lane == 0) mom[q] = s;
     }
     __syncthreads();
-    if (s_ticket != (unsigned)(rowBlocks - 1)) return;
-
-    // ---- last row block: total moments, S, Horn, new (R, t) ----
-    if (wave == 0) {
-        __threadfence();
-        if (lane < kMomW) {
-            double s = 0.0;
-            for (int b = 0; b < rowBlocks; ++b) s += moments[(size_t)b * kMomW + lane];
-            mom[lane] = s;
-        }
+    if (threadIdx.x != 0) return;
+    double* __restrict__ state = d.state;
+    const double L = mom[0];
+    double Xc[3], Yc[3], hS[9], Rn[9], tn[3];
+    for (int i = 0; i < 3; ++i) {
+        Xc[i] = mom[1 + i] / L;
+        Yc[i] = mom[4 + i] / L;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        counters[rowBlocks] = 0u;
-        const double L = mom[0];
-        double Xc[3], Yc[3], hS[9], Rn[9], tn[3];
+    // h_S[3a+b] = sum lambda (y - Yc)_a (X' - Xc)_b   (emicp.cu:621-624)
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) hS[3 * a + b] = mom[7 + 3 * b + a] - L * Xc[b] * Yc[a];
+    const bool ok = horn_rt(Xc, Yc, hS, Rn, tn);
+    if (ok) {
+        for (int i = 0; i < 9; ++i) state[kStR + i] = Rn[i];
+        for (int i = 0; i < 3; ++i) state[kStT + i] = tn[i];
+    } else {
+        state[kStStatus] += 1.0;
+    }
+    state[kStResid] = mom[16] / L;
+    state[kStLambda] = L;
+    if (d.trace) {
+        double* o = d.trace + (size_t)iter * EMSA_TRACE_W;
+        o[0] = sigma_p2;
+        o[1] = L;
         for (int i = 0; i < 3; ++i) {
-            Xc[i] = mom[1 + i] / L;
-            Yc[i] = mom[4 + i] / L;
+            o[2 + i] = Xc[i];
+            o[5 + i] = Yc[i];
         }
-        // h_S[3a+b] = sum lambda (y - Yc)_a (X' - Xc)_b   (emicp.cu:621-624)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) hS[3 * a + b] = mom[7 + 3 * b + a] - L * Xc[b] * Yc[a];
-        const bool ok = horn_rt(Xc, Yc, hS, Rn, tn);
-        if (ok) {
-            for (int i = 0; i < 9; ++i) state[kStR + i] = Rn[i];
-            for (int i = 0; i < 3; ++i) state[kStT + i] = tn[i];
-        } else {
-            state[kStStatus] += 1.0;
-        }
-        state[kStResid] = mom[16] / L;
-        state[kStLambda] = L;
-        if (d.trace) {
-            double* o = d.trace + (size_t)iter * EMSA_TRACE_W;
-            o[0] = sigma_p2;
-            o[1] = L;
-            for (int i = 0; i < 3; ++i) {
-                o[2 + i] = Xc[i];
-                o[5 + i] = Yc[i];
-            }
-            for (int i = 0; i < 9; ++i) o[8 + i] = hS[i];
-            for (int i = 0; i < 9; ++i) o[17 + i] = state[kStR + i];
-            for (int i = 0; i < 3; ++i) o[26 + i] = state[kStT + i];
-            o[29] = mom[16] / L;
-            o[30] = mom[17];
-            o[31] = ok ? 0.0 : 1.0;
-        }
+        for (int i = 0; i < 9; ++i) o[8 + i] = hS[i];
+        for (int i = 0; i < 9; ++i) o[17 + i] = state[kStR + i];
+        for (int i = 0; i < 3; ++i) o[26 + i] = state[kStT + i];
+        o[29] = mom[16] / L;
+        o[30] = mom[17];
+        o[31] = ok ? 0.0 : 1.0;
     }
 }
 
@@ -339,8 +391,8 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
                int tileCols, double* trace, double* rows, hipStream_t s) {
     if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
     w.regs.clear();
-    size_t xpadF = 0, partF = 0, momD = 0, cntU = 0;
-    long long tiles = 0;
+    size_t xpadF = 0, partF = 0, momD = 0;
+    long long tiles = 0, rowBlocks = 0;
     std::vector<size_t> xoff(count);
     for (int k = 0; k < count; ++k) {
         const EmPair& pr = pairs[k];
@@ -371,15 +423,15 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
             d.padOwner = (st == 0);
             d.slot = k * n_starts + st;
             d.tileBase = (int)tiles;
+            d.rowBase = (int)rowBlocks;
             const long long t = (long long)pl.rowBlocks * pl.chunks;
             tiles += t;
+            rowBlocks += pl.rowBlocks;
             // offsets for now, pointers below
             d.partial = reinterpret_cast<float*>(partF);
             partF += (size_t)t * 5 * kEmRows;
             d.moments = reinterpret_cast<double*>(momD);
             momD += (size_t)pl.rowBlocks * kMomW;
-            d.counters = reinterpret_cast<unsigned*>(cntU);
-            cntU += (size_t)pl.rowBlocks + 1;
             d.trace = (count == 1 && n_starts == 1) ? trace : nullptr;
             d.rows = (count == 1 && n_starts == 1) ? rows : nullptr;
             w.regs.push_back(d);
@@ -387,16 +439,12 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     }
     if (tiles > 0x7fffffffLL) return set_error("emicp: batch too large for one launch"), EMSA_EINVAL;
     w.tiles = tiles;
+    w.rowBlocks = rowBlocks;
     const int nregs = (int)w.regs.size();
     int rc;
     if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
     if ((rc = w.partial.reserve(sizeof(float) * partF)) != EMSA_OK) return rc;
     if ((rc = w.moments.reserve(sizeof(double) * momD)) != EMSA_OK) return rc;
-    const size_t oldc = w.counters.bytes;
-    const void* oldp = w.counters.ptr;
-    if ((rc = w.counters.reserve(sizeof(unsigned) * cntU)) != EMSA_OK) return rc;
-    if (w.counters.bytes != oldc || w.counters.ptr != oldp)  // fresh tickets start at zero
-        EMSA_HIP_TRY(hipMemsetAsync(w.counters.ptr, 0, w.counters.bytes, s));
     if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
     if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
     for (int i = 0; i < nregs; ++i) {
@@ -405,7 +453,6 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
         d.Xpad = w.xpad.as<float>() + xoff[k];
         d.partial = w.partial.as<float>() + reinterpret_cast<size_t>(d.partial);
         d.moments = w.moments.as<double>() + reinterpret_cast<size_t>(d.moments);
-        d.counters = w.counters.as<unsigned>() + reinterpret_cast<size_t>(d.counters);
         d.state = w.state.as<double>() + (size_t)i * kStateW;
     }
     EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(EmRegDesc) * nregs,
@@ -421,14 +468,20 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     const EmRegDesc* desc = w.desc.as<EmRegDesc>();
     hipLaunchKernelGGL(emicp_setup_kernel, dim3(nregs), dim3(256), 0, s, desc, dRt0);
     EMSA_HIP_TRY(hipGetLastError());
+    const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
-        hipLaunchKernelGGL(emicp_iter_kernel<true>, dim3((unsigned)w.tiles),
-                           dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk, sig, outlier,
-                           (int)it, last);
+        hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
+                           dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
+        EMSA_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
+                           nregs, w.rowBlocks, outlier, last);
+        EMSA_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,
+                           sig, (int)it);
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(emicp_output_kernel, dim3(nregs), dim3(64), 0, s, desc, dRtOut, dResid);

@@ -67,16 +67,15 @@ struct EmRegDesc {
     const float* Y;      // source, SoA [3][M]
     double* state;       // kStateW doubles
     float* partial;      // rowBlocks*chunks*5*kEmRows chunk partial sums
-    double* moments;     // rowBlocks*kMomW row-block moments
-    unsigned* counters;  // rowBlocks + 1 arrival tickets
+    double* moments;     // kMomW x rowBlocks row-block moments (moment-major)
     double* trace;       // nullable: EMSA_TRACE_W doubles per sigma step
     double* rows;        // nullable: C_r, lambda_r (2*M) of the last step
     double floorA;       // (double)(1.0f / N), the floored affinity
     int N, M, ldx, rowBlocks, chunks, waveCols;
-    int tileBase;        // first workgroup index of this registration
+    int tileBase;        // first affinity workgroup of this registration
+    int rowBase;         // first row block (over the whole table) of this registration
     int padOwner;        // 1: this registration writes Xpad (once per target cloud)
     int slot;            // output slot
-    int pad_;
 };
 
 // One source/target pair of an EM-ICP batch (device or host pointers,
@@ -90,9 +89,10 @@ struct EmPair {
 
 // Workspace and descriptor table of one EM-ICP batch.
 struct EmWork {
-    DevBuf xpad, partial, moments, counters, state, desc;
+    DevBuf xpad, partial, moments, state, desc;
     std::vector<EmRegDesc> regs;
-    long long tiles = 0;
+    long long tiles = 0;      // affinity workgroups per step
+    long long rowBlocks = 0;  // row blocks per step over the whole table
 };
 
 struct GraphEntry {

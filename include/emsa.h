@@ -145,7 +145,8 @@ int emsa_emicp_batched_dev(const emsa_pair *pairs, int count, const float *dRt0,
 /* Autotuner: times the candidate work decompositions on these device buffers
  * (the first sigma^2 steps of p) and keeps the fastest for later calls on the
  * current device. count == 1 && n_starts == 1 tunes the single-registration
- * plan (workgroups per CU), otherwise the batched column tile width. The
+ * plan (workgroups per CU, from 2..32), otherwise the batched column tile width
+ * (512..8192). The
  * choice is written to *choice (nullable). Results of different choices
  * differ only by FP32 summation order. */
 int emsa_autotune_emicp(const emsa_pair *pairs, int count, int n_starts, const emsa_params *p,
