@@ -68,6 +68,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "emsa_graph_cache_clear": (C.c_int, []),
     "emsa_set_tiling": (C.c_int, [C.c_int, C.c_int]),
+    "emsa_probe_emicp_affinity": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_int, C.c_float,
+                                            C.c_int, _dp, _dp]),
 }
 
 _lib = None

@@ -11,7 +11,7 @@ namespace emsa {
 
 namespace {
 
-constexpr size_t kMaxGraphs = 4;
+constexpr size_t kMaxGraphs = 32;
 constexpr int kDefaultTileCols = 2048;
 uint64_t g_clock = 0;
 
@@ -40,7 +40,7 @@ void drop_entry(GraphEntry& e) {
     e.work.desc.release();
 }
 
-int tile_cols(const DeviceCtx& ctx) { return ctx.emTileCols > 0 ? ctx.emTileCols : kDefaultTileCols; }
+int tile_cols(const DeviceCtx& ctx) { return default_tile_cols(ctx); }
 
 // Finds or builds the captured launch sequence for this call and launches it
 // on `user`. The sequence is recorded on the library's own stream (the null
@@ -104,6 +104,10 @@ int check_pairs(const emsa_pair* pairs, int count) {
 }
 
 }  // namespace
+
+int default_tile_cols(const DeviceCtx& ctx) {
+    return ctx.emTileCols > 0 ? ctx.emTileCols : kDefaultTileCols;
+}
 
 }  // namespace emsa
 

@@ -489,6 +489,22 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     return EMSA_OK;
 }
 
+int em_launch_affinity(EmWork& w, float sigma_p2, hipStream_t s) {
+    const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
+    hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
+                       dim3(kEmRows * kEmWaves), 0, s, w.desc.as<EmRegDesc>(),
+                       (int)w.regs.size(), negk);
+    EMSA_HIP_TRY(hipGetLastError());
+    return EMSA_OK;
+}
+
+int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s) {
+    hipLaunchKernelGGL(emicp_setup_kernel, dim3((unsigned)w.regs.size()), dim3(256), 0, s,
+                       w.desc.as<EmRegDesc>(), dRt0);
+    EMSA_HIP_TRY(hipGetLastError());
+    return EMSA_OK;
+}
+
 int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                  const float* dRtIn, float* dRtOut, const emsa_params& p,
                  const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows) {

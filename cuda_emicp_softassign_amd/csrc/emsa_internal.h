@@ -112,7 +112,7 @@ struct DeviceCtx {
     DevBuf X, Y, rt, trace, rows, misc, resid;
     EmWork em;
     // softassign workspaces
-    DevBuf K, u, v, orow, ocol, rowsum, colsum, colpart, rowpart, sacnt, samom, sastate;
+    DevBuf K, u, v, orow, ocol, colpart, samom, sastate;
     // captured EM-ICP launch sequences of the device entry points, replayed
     // when a call repeats the same buffers, sizes and parameters
     std::vector<std::unique_ptr<GraphEntry>> graphs;
@@ -140,6 +140,11 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
 // dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.
 int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
                const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s);
+
+// Single launches of the setup and affinity kernels (measurement probes).
+int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s);
+int em_launch_affinity(EmWork& w, float sigma_p2, hipStream_t s);
+int default_tile_cols(const DeviceCtx& ctx);
 
 // Runs the EM-ICP loop on device buffers for one registration (eager).
 int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,

@@ -7,6 +7,8 @@
 
 #include "../../include/emsa_probe.h"
 
+#include <vector>
+
 namespace emsa {
 namespace {
 
@@ -82,5 +84,53 @@ extern "C" int emsa_probe_rates(double* exp_per_s, double* fma_per_s) {
     *fma_per_s = work / (ms * 1e-3);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return EMSA_OK;
+}
+
+extern "C" int emsa_probe_emicp_affinity(const emsa_pair* pairs, int count, int n_starts,
+                                         float sigma_p2, int reps, double* us_per_launch,
+                                         double* pairs_per_launch) {
+    clear_error();
+    if (!pairs || count <= 0 || n_starts <= 0 || reps <= 0 || !us_per_launch ||
+        !pairs_per_launch || !(sigma_p2 > 0.f))
+        return set_error("probe_emicp_affinity: bad arguments"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    std::vector<EmPair> pv(count);
+    double np = 0.0;
+    for (int k = 0; k < count; ++k) {
+        if (!pairs[k].X || !pairs[k].Y || pairs[k].N <= 0 || pairs[k].M <= 0)
+            return set_error("probe_emicp_affinity: null or empty cloud"), EMSA_EINVAL;
+        pv[k] = EmPair{pairs[k].X, pairs[k].N, pairs[k].Y, pairs[k].M};
+        np += (double)pairs[k].N * pairs[k].M * n_starts;
+    }
+    const bool single = (count == 1 && n_starts == 1);
+    int rc = em_prepare(*ctx, ctx->em, pv.data(), count, n_starts,
+                        single ? 0 : default_tile_cols(*ctx), nullptr, nullptr, s);
+    if (rc != EMSA_OK) return rc;
+    const size_t nregs = (size_t)count * n_starts;
+    if ((rc = ctx->rt.reserve(sizeof(float) * 12 * nregs)) != EMSA_OK) return rc;
+    std::vector<float> rt0(12 * nregs, 0.f);
+    for (size_t i = 0; i < nregs; ++i) rt0[12 * i] = rt0[12 * i + 4] = rt0[12 * i + 8] = 1.f;
+    EMSA_HIP_TRY(hipMemcpyAsync(ctx->rt.ptr, rt0.data(), sizeof(float) * rt0.size(),
+                                hipMemcpyHostToDevice, s));
+    if ((rc = em_launch_setup(ctx->em, ctx->rt.as<float>(), s)) != EMSA_OK) return rc;
+    if ((rc = em_launch_affinity(ctx->em, sigma_p2, s)) != EMSA_OK) return rc;  // warm-up
+    hipEvent_t e0, e1;
+    EMSA_HIP_TRY(hipEventCreate(&e0));
+    EMSA_HIP_TRY(hipEventCreate(&e1));
+    EMSA_HIP_TRY(hipEventRecord(e0, s));
+    for (int i = 0; i < reps && rc == EMSA_OK; ++i) rc = em_launch_affinity(ctx->em, sigma_p2, s);
+    EMSA_HIP_TRY(hipEventRecord(e1, s));
+    EMSA_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    EMSA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != EMSA_OK) return rc;
+    *us_per_launch = 1e3 * (double)ms / reps;
+    *pairs_per_launch = np;
     return EMSA_OK;
 }

@@ -15,6 +15,18 @@ extern "C" {
  * lane per second, i.e. 2 flops each). Returns EMSA_OK or a negative code. */
 int emsa_probe_rates(double *exp_per_s, double *fma_per_s);
 
+/* Average duration in microseconds of one launch of the fused EM-ICP
+ * affinity kernel over the batch `pairs` (device SoA pointers) x n_starts
+ * identity starts, laid out exactly as emsa_emicp_batched_dev lays it out
+ * (count == 1 && n_starts == 1: the single-registration plan of
+ * emsa_emicp_dev), timed with HIP events over `reps` back-to-back launches on
+ * the library's stream at the given sigma^2. *pairs_per_launch receives the
+ * number of (target, source) pairs one launch evaluates. */
+struct emsa_pair;
+int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_starts,
+                              float sigma_p2, int reps, double *us_per_launch,
+                              double *pairs_per_launch);
+
 #ifdef __cplusplus
 }
 #endif
