@@ -204,7 +204,7 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 // Stage 2: one wave per row block (64 source rows).
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
-                  double outlier, int last) {
+                  double outlier, int last, int iter) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long gr = (long long)blockIdx.x * kEmWaves + wave;
     if (gr >= totalRowBlocks) return;  // wave-uniform
@@ -271,6 +271,14 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     if (last && d.rows && valid) {
         d.rows[r] = Cs;
         d.rows[(size_t)M + r] = lam;
+    }
+    if (d.ckpt && valid) {
+        double* ck = d.ckpt + ((size_t)iter * M + r) * 5;
+        ck[0] = Cs;
+        ck[1] = lam;
+        ck[2] = Vx;
+        ck[3] = Vy;
+        ck[4] = Vz;
     }
     double pxd, pyd, pzd;
     transform_row(state, yx, yy, yz, pxd, pyd, pzd);
@@ -388,7 +396,7 @@ std::vector<float> sigma_schedule(const emsa_params& p) {
 }
 
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
-               int tileCols, double* trace, double* rows, hipStream_t s) {
+               int tileCols, double* trace, double* rows, hipStream_t s, const EmAux* aux) {
     if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
     w.regs.clear();
     size_t xpadF = 0, partF = 0, momD = 0;
@@ -434,6 +442,11 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
             momD += (size_t)pl.rowBlocks * kMomW;
             d.trace = (count == 1 && n_starts == 1) ? trace : nullptr;
             d.rows = (count == 1 && n_starts == 1) ? rows : nullptr;
+            d.ckpt = nullptr;
+            if (aux) {
+                d.trace = aux[k * n_starts + st].trace;
+                d.ckpt = aux[k * n_starts + st].ckpt;
+            }
             w.regs.push_back(d);
         }
     }
@@ -478,7 +491,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
                            dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
         EMSA_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
-                           nregs, w.rowBlocks, outlier, last);
+                           nregs, w.rowBlocks, outlier, last, (int)it);
         EMSA_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,
                            sig, (int)it);

@@ -70,6 +70,7 @@ struct EmRegDesc {
     double* moments;     // kMomW x rowBlocks row-block moments (moment-major)
     double* trace;       // nullable: EMSA_TRACE_W doubles per sigma step
     double* rows;        // nullable: C_r, lambda_r (2*M) of the last step
+    double* ckpt;        // nullable: [steps][M][5] C_r, lambda_r, lambda_r X'_r per step
     double floorA;       // (double)(1.0f / N), the floored affinity
     int N, M, ldx, rowBlocks, chunks, waveCols;
     int tileBase;        // first affinity workgroup of this registration
@@ -133,8 +134,15 @@ EmPlan plan_emicp(int N, int M, int numCU, int chunks_override = 0);
 // (allocating its buffers and uploading the descriptors on stream s).
 // tileCols > 0: fixed column tile per workgroup (batched); 0: the single-
 // registration plan that spreads one registration over the whole device.
+// aux (nullable): per registration, a trace base ([steps][EMSA_TRACE_W]) and a
+// checkpoint base ([steps][M][5]) for the unrolled (differentiable) forward.
+struct EmAux {
+    double* trace;
+    double* ckpt;
+};
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
-               int tileCols, double* trace, double* rows, hipStream_t s);
+               int tileCols, double* trace, double* rows, hipStream_t s,
+               const EmAux* aux = nullptr);
 
 // Enqueues setup, one launch per sigma^2 step, and the output kernel.
 // dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.
