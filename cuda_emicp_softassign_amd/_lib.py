@@ -41,6 +41,11 @@ class emsa_pair(C.Structure):
     _fields_ = [("X", C.c_void_p), ("N", C.c_int), ("Y", C.c_void_p), ("M", C.c_int)]
 
 
+class emsa_pair_grad(C.Structure):
+    """Mirror of struct emsa_pair_grad (include/emsa.h)."""
+    _fields_ = [("gX", C.c_void_p), ("gY", C.c_void_p)]
+
+
 # Every function the header declares, with its ctypes signature.
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -70,6 +75,14 @@ SIGNATURES = {
     "emsa_set_tiling": (C.c_int, [C.c_int, C.c_int]),
     "emsa_probe_emicp_affinity": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_int, C.c_float,
                                             C.c_int, _dp, _dp]),
+    "emsa_emicp_unrolled_ckpt_size": (C.c_longlong, [C.POINTER(emsa_pair), C.c_int, C.c_int]),
+    "emsa_emicp_unrolled_fwd": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_void_p, C.c_float,
+                                          C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
+    "emsa_emicp_unrolled_bwd": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_void_p, C.c_float,
+                                          C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.POINTER(emsa_pair_grad), C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
 }
 
 _lib = None

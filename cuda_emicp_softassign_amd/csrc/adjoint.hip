@@ -1,0 +1,696 @@
+// Differentiable unrolled EM-ICP: forward with per-step checkpoints and the
+// hand-written adjoint (no counterpart in the reference; it differentiates the
+// iteration of src/emicp.cu:443-668).
+//
+// Forward (steps k = 0..K-1, sigma_k^2 = sigma_0^2 f^k): the ordinary fused
+// kernels, with every registration also storing per step its trace record
+// (Lambda, Xc, Yc, S, R_{k+1}, t_{k+1}) and per source row (C_r, lambda_r,
+// lambda_r X'_r). No affinity is stored.
+//
+// Backward, per step from K-1 down to 0, given dL/dR_{k+1}, dL/dt_{k+1}:
+//  1. adj_solve: Horn's step backwards -- t = Xc - R Yc, R = R(q), q the top
+//     eigenvector of N(S) (dq = sum_{j != top} v_j v_j^T dN q / (w_top - w_j)),
+//     S = sum lambda X' y^T - Lambda Xc Yc^T, Xc = SX / Lambda, Yc = SY / Lambda
+//     -- giving the 16 moment coefficients dL/d(Lambda, SX, SY, SXY).
+//  2. adj_pairs: per source row, dL/dB, dL/dC, dL/dW from lambda = B / sqrt(C),
+//     lambda X' = W / sqrt(C); then every pair recomputes
+//     e = exp(-|x - p|^2 / (2 sigma^2)) (compensated difference, as the forward)
+//     and accumulates g_e = dB + 2 e dC + dW . x, h = -g_e e / sigma^2:
+//     dL/dp_r -= h d, dL/dx_c += h d + e dW, dL/dsigma^2 += g_e e |d|^2 / (2 sigma^4).
+//     Lanes hold 4 source rows each, so the per-column cross-lane sums are
+//     shared by 256 pairs.
+//  3. adj_rows: per row, dL/dy_r = R^T dL/dp_r + the direct moment terms, and
+//     the registration partials dL/dR_k = sum dp y^T, dL/dt_k = sum dp.
+//  4. adj_finish: per registration, the new (dL/dR_k, dL/dt_k) and
+//     dL/dsigma_0^2 += f^k dL/dsigma_k^2.
+//  5. adj_cols: per target column, dL/dx_c summed over row tiles in a fixed
+//     order (+ the floored rows' 1/N share).
+// All reductions have a fixed order; no workgroup waits on another.
+#include "emsa_device.h"
+#include "emsa_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace emsa {
+
+namespace {
+
+constexpr int kAdjRPL = 4;                    // source rows per lane
+constexpr int kAdjRows = kAdjRPL * kWave;      // 256 rows per tile (one wave covers them)
+constexpr int kAdjWaves = 4;                   // waves per workgroup split the column chunk
+constexpr int kAdjMomW = 16;                   // per-tile partials: dR(9) dt(3) dsig(1) Gfl(3)
+constexpr int kAdjStW = 24;                    // adjoint state: dR(9) dt(3) dsig0(1) Gfl(3) ...
+constexpr float kSentinelA = 1.0e15f;
+typedef const __attribute__((address_space(4))) float* cfloat_p;
+
+struct AdjDesc {
+    const float* X;     // target, SoA [3][N]
+    float* Xpad;        // target, SoA [3][ldx], padded
+    const float* Y;     // source, SoA [3][M]
+    const double* rec;  // forward records [steps][EMSA_TRACE_W]
+    const double* ck;   // forward row checkpoints [steps][M][5]
+    const float* Rt0;   // initial transform, 12 floats
+    double* adj;        // adjoint state, kAdjStW doubles
+    double* coef;       // 16 moment coefficients of the current step
+    float* colpart;     // [rowTiles][ldx][3]
+    float* rowpart;     // [rowTiles * chunks][4][kAdjRows]
+    double* rtmom;      // [rowTiles][kAdjMomW]
+    float* gX;          // [3][N] accumulated over steps
+    float* gY;          // [3][M] accumulated over steps
+    double floorA;
+    int N, M, ldx, rowTiles, chunks, waveCols, tileBase, rowBase, colBase, pad_;
+};
+
+template <class F>
+__device__ inline int find_by(const AdjDesc* __restrict__ d, int n, int b, F key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(d[mid]) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// (R_k, t_k): the transform step k started from.
+__device__ inline void step_input(const AdjDesc& d, int k, double* R, double* t) {
+    if (k == 0) {
+        for (int i = 0; i < 9; ++i) R[i] = d.Rt0[i];
+        for (int i = 0; i < 3; ++i) t[i] = d.Rt0[9 + i];
+    } else {
+        const double* o = d.rec + (size_t)(k - 1) * EMSA_TRACE_W;
+        for (int i = 0; i < 9; ++i) R[i] = o[17 + i];
+        for (int i = 0; i < 3; ++i) t[i] = o[26 + i];
+    }
+}
+
+// Full cyclic Jacobi: A (destroyed) -> eigenvalues w, eigenvectors columns of V.
+__device__ inline void jacobi4_full(double A[4][4], double V[4][4], double w[4]) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            diag += A[i][i] * A[i][i];
+            for (int j = i + 1; j < 4; ++j) off += A[i][j] * A[i][j];
+        }
+        if (off <= 1e-36 * diag || off == 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int r = p + 1; r < 4; ++r) {
+                const double apq = A[p][r];
+                if (apq == 0.0) continue;
+                const double tau = (A[r][r] - A[p][p]) / (2.0 * apq);
+                const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
+                for (int k = 0; k < 4; ++k) {
+                    const double akp = A[k][p], akq = A[k][r];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][r] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double apk = A[p][k], aqk = A[r][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[r][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][r];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][r] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int k = 0; k < 4; ++k) w[k] = A[k][k];
+}
+
+// Horn's step backwards (see the file comment). gR (row-major), gt: the
+// gradient of the step's outputs. coef = dL/d of the step's moments in their
+// forward order: [Lambda, SX(3), SY(3), SXY(9)], SXY[i][j] = sum lambda X'_i y_j
+// at 7 + 3i + j.
+__device__ inline void horn_backward(const double* rec, const double* gR, const double* gt,
+                                     double* coef) {
+    const double L = rec[1];
+    const double Xc[3] = {rec[2], rec[3], rec[4]};
+    const double Yc[3] = {rec[5], rec[6], rec[7]};
+    const double* S = rec + 8;  // h_S[3a+b] = sum (source)_a (target)_b
+    for (int i = 0; i < 16; ++i) coef[i] = 0.0;
+    if (rec[31] != 0.0 || !(L > 0.0)) return;  // the forward step failed: no gradient
+    double A[4][4];
+    A[0][0] = S[0] + S[4] + S[8];
+    A[0][1] = S[5] - S[7];
+    A[0][2] = S[6] - S[2];
+    A[0][3] = S[1] - S[3];
+    A[1][1] = S[0] - S[4] - S[8];
+    A[1][2] = S[1] + S[3];
+    A[1][3] = S[6] + S[2];
+    A[2][2] = S[4] - S[0] - S[8];
+    A[2][3] = S[5] + S[7];
+    A[3][3] = S[8] - S[0] - S[4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < i; ++j) A[i][j] = A[j][i];
+    double V[4][4], w[4];
+    jacobi4_full(A, V, w);
+    int top = 0;
+    for (int k = 1; k < 4; ++k)
+        if (w[k] > w[top]) top = k;
+    double q[4];
+    for (int i = 0; i < 4; ++i) q[i] = V[i][top];
+    const double q0 = q[0], qx = q[1], qy = q[2], qz = q[3];
+    const double R[9] = {q0 * q0 + qx * qx - qy * qy - qz * qz, 2 * (qx * qy - q0 * qz),
+                         2 * (qx * qz + q0 * qy),           2 * (qy * qx + q0 * qz),
+                         q0 * q0 - qx * qx + qy * qy - qz * qz, 2 * (qy * qz - q0 * qx),
+                         2 * (qz * qx - q0 * qy),           2 * (qz * qy + q0 * qx),
+                         q0 * q0 - qx * qx - qy * qy + qz * qz};
+    // t = Xc - R Yc
+    double gXc[3], gYc[3], G[9];
+    for (int i = 0; i < 3; ++i) gXc[i] = gt[i];
+    for (int j = 0; j < 3; ++j) gYc[j] = -(R[j] * gt[0] + R[3 + j] * gt[1] + R[6 + j] * gt[2]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) G[3 * i + j] = gR[3 * i + j] - gt[i] * Yc[j];
+    // R = R(q)
+    double gq[4];
+    gq[0] = 2 * (q0 * G[0] - qz * G[1] + qy * G[2] + qz * G[3] + q0 * G[4] - qx * G[5] -
+                 qy * G[6] + qx * G[7] + q0 * G[8]);
+    gq[1] = 2 * (qx * G[0] + qy * G[1] + qz * G[2] + qy * G[3] - qx * G[4] - q0 * G[5] +
+                 qz * G[6] + q0 * G[7] - qx * G[8]);
+    gq[2] = 2 * (-qy * G[0] + qx * G[1] + q0 * G[2] + qx * G[3] + qy * G[4] + qz * G[5] -
+                 q0 * G[6] + qz * G[7] - qy * G[8]);
+    gq[3] = 2 * (-qz * G[0] - q0 * G[1] + qx * G[2] + q0 * G[3] - qz * G[4] + qy * G[5] +
+                 qx * G[6] + qy * G[7] + qz * G[8]);
+    // q = top eigenvector of A: gA = sum_{j != top} (v_j . gq) / (w_top - w_j) v_j q^T
+    double gA[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) gA[a][b] = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        if (j == top) continue;
+        const double gap = w[top] - w[j];
+        if (gap == 0.0) continue;
+        double dot = 0.0;
+        for (int i = 0; i < 4; ++i) dot += V[i][j] * gq[i];
+        const double c = dot / gap;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) gA[a][b] += c * V[a][j] * q[b];
+    }
+    // A = N(S): gS[m] = sum_{ab} gA[a][b] dA_ab / dS_m
+    double gS[9];
+    const double s01 = gA[0][1] + gA[1][0], s02 = gA[0][2] + gA[2][0], s03 = gA[0][3] + gA[3][0];
+    const double s12 = gA[1][2] + gA[2][1], s13 = gA[1][3] + gA[3][1], s23 = gA[2][3] + gA[3][2];
+    const double d0 = gA[0][0], d1 = gA[1][1], d2 = gA[2][2], d3 = gA[3][3];
+    gS[0] = d0 + d1 - d2 - d3;
+    gS[4] = d0 - d1 + d2 - d3;
+    gS[8] = d0 - d1 - d2 + d3;
+    gS[5] = s01 + s23;
+    gS[7] = -s01 + s23;
+    gS[6] = s02 + s13;
+    gS[2] = -s02 + s13;
+    gS[1] = s03 + s12;
+    gS[3] = -s03 + s12;
+    // h_S[3j+i] = SXY[i][j] - L Xc_i Yc_j
+    double gL = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double g = gS[3 * j + i];
+            coef[7 + 3 * i + j] = g;  // dL/dSXY[i][j]
+            gXc[i] -= L * g * Yc[j];
+            gYc[j] -= L * g * Xc[i];
+            gL -= g * Xc[i] * Yc[j];
+        }
+    // Xc = SX / L, Yc = SY / L
+    for (int i = 0; i < 3; ++i) {
+        coef[1 + i] = gXc[i] / L;
+        coef[4 + i] = gYc[i] / L;
+        gL -= (gXc[i] * Xc[i] + gYc[i] * Yc[i]) / L;
+    }
+    coef[0] = gL;
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+adj_setup_kernel(const AdjDesc* __restrict__ regs, const float* __restrict__ gRt) {
+    const AdjDesc& d = regs[blockIdx.x];
+    const int N = d.N, M = d.M, ldx = d.ldx;
+    for (int c = threadIdx.x; c < ldx; c += blockDim.x) {
+        float x = kSentinelA, y = kSentinelA, z = kSentinelA;
+        if (c < N) {
+            x = d.X[c];
+            y = d.X[(size_t)N + c];
+            z = d.X[2 * (size_t)N + c];
+            d.gX[c] = 0.f;
+            d.gX[(size_t)N + c] = 0.f;
+            d.gX[2 * (size_t)N + c] = 0.f;
+        }
+        d.Xpad[c] = x;
+        d.Xpad[(size_t)ldx + c] = y;
+        d.Xpad[2 * (size_t)ldx + c] = z;
+    }
+    for (int r = threadIdx.x; r < 3 * M; r += blockDim.x) d.gY[r] = 0.f;
+    if (threadIdx.x < kAdjStW) {
+        const int i = threadIdx.x;
+        d.adj[i] = (i < 12) ? (double)gRt[(size_t)blockIdx.x * 12 + i] : 0.0;
+    }
+}
+
+__global__ void adj_solve_kernel(const AdjDesc* __restrict__ regs, int k) {
+    if (threadIdx.x != 0) return;
+    const AdjDesc& d = regs[blockIdx.x];
+    horn_backward(d.rec + (size_t)k * EMSA_TRACE_W, d.adj, d.adj + 9, d.coef);
+}
+
+// Per-row coefficients of one source row for step k.
+struct RowCoef {
+    double gB, gC, gW[3], gLam, gV[3], lam, V[3];
+    bool floored;
+};
+
+__device__ inline RowCoef row_coef(const AdjDesc& d, const double* coef, int k, int r, float yx,
+                                   float yy, float yz) {
+    RowCoef o;
+    const double* ck = d.ck + ((size_t)k * d.M + r) * 5;
+    const double C = ck[0];
+    o.lam = ck[1];
+    o.V[0] = ck[2];
+    o.V[1] = ck[3];
+    o.V[2] = ck[4];
+    const double y[3] = {yx, yy, yz};
+    o.gLam = coef[0] + coef[4] * y[0] + coef[5] * y[1] + coef[6] * y[2];
+    for (int i = 0; i < 3; ++i)
+        o.gV[i] = coef[1 + i] + coef[7 + 3 * i] * y[0] + coef[8 + 3 * i] * y[1] +
+                  coef[9 + 3 * i] * y[2];
+    o.floored = !((float)C > 10e-7f);
+    if (o.floored) {
+        o.gB = o.gC = 0.0;
+        o.gW[0] = o.gW[1] = o.gW[2] = 0.0;
+    } else {
+        const double rs = 1.0 / sqrt(C);
+        o.gB = o.gLam * rs;
+        for (int i = 0; i < 3; ++i) o.gW[i] = o.gV[i] * rs;
+        o.gC = -(o.lam * o.gLam + o.V[0] * o.gV[0] + o.V[1] * o.gV[1] + o.V[2] * o.gV[2]) /
+               (2.0 * C);
+    }
+    return o;
+}
+
+// Stage 2: a workgroup = 256 source rows x one column chunk; lanes hold 4
+// rows each, the 4 waves split the chunk.
+__global__ void __launch_bounds__(kWave * kAdjWaves)
+adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk, float minv_s2) {
+    const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.tileBase; });
+    const AdjDesc& d = regs[ri];
+    const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
+    const int local = blockIdx.x - d.tileBase;
+    const int rt = local / chunks;
+    const int g = local - rt * chunks;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    double Rk[9], tk[3];
+    step_input(d, k, Rk, tk);
+    double coef[16];
+    for (int i = 0; i < 16; ++i) coef[i] = d.coef[i];
+
+    float px[kAdjRPL], py[kAdjRPL], pz[kAdjRPL], lx[kAdjRPL], ly[kAdjRPL], lz[kAdjRPL];
+    float gB[kAdjRPL], gC2[kAdjRPL], gWx[kAdjRPL], gWy[kAdjRPL], gWz[kAdjRPL];
+#pragma unroll
+    for (int j = 0; j < kAdjRPL; ++j) {
+        const int r = rt * kAdjRows + j * kWave + lane;
+        float yx = 0.f, yy = 0.f, yz = 0.f;
+        if (r < M) {
+            yx = d.Y[r];
+            yy = d.Y[(size_t)M + r];
+            yz = d.Y[2 * (size_t)M + r];
+        }
+        const double pxd = Rk[0] * yx + Rk[1] * yy + Rk[2] * yz + tk[0];
+        const double pyd = Rk[3] * yx + Rk[4] * yy + Rk[5] * yz + tk[1];
+        const double pzd = Rk[6] * yx + Rk[7] * yy + Rk[8] * yz + tk[2];
+        px[j] = (float)pxd;
+        py[j] = (float)pyd;
+        pz[j] = (float)pzd;
+        lx[j] = (float)(pxd - (double)px[j]);
+        ly[j] = (float)(pyd - (double)py[j]);
+        lz[j] = (float)(pzd - (double)pz[j]);
+        gB[j] = gC2[j] = gWx[j] = gWy[j] = gWz[j] = 0.f;
+        if (r < M) {
+            const RowCoef o = row_coef(d, coef, k, r, yx, yy, yz);
+            gB[j] = (float)o.gB;
+            gC2[j] = (float)(2.0 * o.gC);
+            gWx[j] = (float)o.gW[0];
+            gWy[j] = (float)o.gW[1];
+            gWz[j] = (float)o.gW[2];
+        }
+    }
+
+    const cfloat_p Xx = (cfloat_p)d.Xpad;
+    const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
+    const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
+    const int cb = (g * kAdjWaves + wave) * waveCols;
+    const int ce = min(cb + waveCols, ldx);
+    float gpx[kAdjRPL], gpy[kAdjRPL], gpz[kAdjRPL], gs[kAdjRPL];
+#pragma unroll
+    for (int j = 0; j < kAdjRPL; ++j) gpx[j] = gpy[j] = gpz[j] = gs[j] = 0.f;
+    float* __restrict__ colpart = d.colpart + ((size_t)rt * ldx) * 3;
+    for (int c = cb; c < ce; c += kColGroup) {
+#pragma unroll 4
+        for (int u = 0; u < kColGroup; ++u) {
+            const float xx = Xx[c + u], xy = Xy[c + u], xz = Xz[c + u];
+            float cx = 0.f, cy = 0.f, cz = 0.f;
+#pragma unroll
+            for (int j = 0; j < kAdjRPL; ++j) {
+                const float dx = (xx - px[j]) - lx[j];
+                const float dy = (xy - py[j]) - ly[j];
+                const float dz = (xz - pz[j]) - lz[j];
+                float s = dx * dx;
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                const float e = __builtin_amdgcn_exp2f(s * negk);
+                const float ge = fmaf(gWz[j], xz, fmaf(gWy[j], xy, fmaf(gWx[j], xx, fmaf(e, gC2[j], gB[j]))));
+                const float t1 = ge * e;
+                const float h = t1 * minv_s2;
+                gpx[j] = fmaf(h, dx, gpx[j]);
+                gpy[j] = fmaf(h, dy, gpy[j]);
+                gpz[j] = fmaf(h, dz, gpz[j]);
+                gs[j] = fmaf(t1, s, gs[j]);
+                cx = fmaf(h, dx, fmaf(e, gWx[j], cx));
+                cy = fmaf(h, dy, fmaf(e, gWy[j], cy));
+                cz = fmaf(h, dz, fmaf(e, gWz[j], cz));
+            }
+            cx = wave_sum(cx);
+            cy = wave_sum(cy);
+            cz = wave_sum(cz);
+            if (lane == 0) {
+                float* o = colpart + (size_t)(c + u) * 3;
+                o[0] = cx;
+                o[1] = cy;
+                o[2] = cz;
+            }
+        }
+    }
+    // row partials: combine the 4 waves (fixed order)
+    __shared__ float red[kAdjWaves][4][kAdjRows];
+#pragma unroll
+    for (int j = 0; j < kAdjRPL; ++j) {
+        red[wave][0][j * kWave + lane] = gpx[j];
+        red[wave][1][j * kWave + lane] = gpy[j];
+        red[wave][2][j * kWave + lane] = gpz[j];
+        red[wave][3][j * kWave + lane] = gs[j];
+    }
+    __syncthreads();
+    float* P = d.rowpart + (size_t)local * 4 * kAdjRows;
+    for (int q = 0; q < 4; ++q) {
+        float acc = red[0][q][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kAdjWaves; ++w) acc += red[w][q][threadIdx.x];
+        P[q * kAdjRows + threadIdx.x] = acc;
+    }
+}
+
+// Stage 3: one workgroup per 256-row tile, one thread per row.
+__global__ void __launch_bounds__(kAdjRows)
+adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double inv2s4,
+                double outlier_dsig) {
+    const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.rowBase; });
+    const AdjDesc& d = regs[ri];
+    const int M = d.M, chunks = d.chunks;
+    const int rt = blockIdx.x - d.rowBase;
+    const int r = rt * kAdjRows + threadIdx.x;
+    double m[kAdjMomW];
+#pragma unroll
+    for (int i = 0; i < kAdjMomW; ++i) m[i] = 0.0;
+    if (r < M) {
+        double gp[3] = {0.0, 0.0, 0.0}, gsum = 0.0;
+        const float* P = d.rowpart + (size_t)rt * chunks * 4 * kAdjRows + threadIdx.x;
+        for (int c = 0; c < chunks; ++c) {
+            const float* q = P + (size_t)c * 4 * kAdjRows;
+            gp[0] += q[0];
+            gp[1] += q[kAdjRows];
+            gp[2] += q[2 * kAdjRows];
+            gsum += q[3 * kAdjRows];
+        }
+        // accumulated +h d; dL/dp = -sum h d
+        gp[0] = -gp[0];
+        gp[1] = -gp[1];
+        gp[2] = -gp[2];
+        const float yx = d.Y[r], yy = d.Y[(size_t)M + r], yz = d.Y[2 * (size_t)M + r];
+        const RowCoef o = row_coef(d, d.coef, k, r, yx, yy, yz);
+        double Rk[9], tk[3];
+        step_input(d, k, Rk, tk);
+        const double y[3] = {yx, yy, yz};
+        double gy[3];
+        for (int j = 0; j < 3; ++j) {
+            // moments: lambda y_j and V_i y_j
+            double g = o.lam * d.coef[4 + j];
+            for (int i = 0; i < 3; ++i) g += d.coef[7 + 3 * i + j] * o.V[i];
+            // p = R y + t
+            g += Rk[j] * gp[0] + Rk[3 + j] * gp[1] + Rk[6 + j] * gp[2];
+            gy[j] = g;
+        }
+        d.gY[r] += (float)gy[0];
+        d.gY[(size_t)M + r] += (float)gy[1];
+        d.gY[2 * (size_t)M + r] += (float)gy[2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) m[3 * i + j] = gp[i] * y[j];
+        for (int i = 0; i < 3; ++i) m[9 + i] = gp[i];
+        m[12] = gsum * inv2s4 + (o.floored ? 0.0 : o.gC * outlier_dsig);
+        if (o.floored)
+            for (int i = 0; i < 3; ++i) m[13 + i] = o.gV[i];
+    }
+    __shared__ double lds[4 * kAdjMomW];
+    double tot;
+    block_sum<kAdjMomW>(m, lds, kAdjRows / kWave, tot);
+    if (threadIdx.x < kAdjMomW) d.rtmom[(size_t)rt * kAdjMomW + threadIdx.x] = tot;
+}
+
+// Stage 4: one workgroup per registration.
+__global__ void __launch_bounds__(64)
+adj_finish_kernel(const AdjDesc* __restrict__ regs, int k, double fk) {
+    const AdjDesc& d = regs[blockIdx.x];
+    const int i = threadIdx.x;
+    if (i >= kAdjMomW) return;
+    // a failed Horn solve left (R, t) unchanged: its gradient passes through
+    const bool failed = d.rec[(size_t)k * EMSA_TRACE_W + 31] != 0.0;
+    double s = 0.0;
+    for (int t = 0; t < d.rowTiles; ++t) s += d.rtmom[(size_t)t * kAdjMomW + i];
+    if (i < 12) {
+        if (!failed) d.adj[i] = s;              // dL/dR_k, dL/dt_k
+    }
+    else if (i == 12) d.adj[12] += fk * s;      // dL/dsigma_0^2
+    else d.adj[i] = s;                          // 13..15: floored rows' dL/dV sum
+}
+
+// Stage 5: one thread per target column.
+__global__ void __launch_bounds__(256)
+adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols) {
+    const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gc >= totalCols) return;
+    const int ri = find_by(regs, nregs, (int)gc, [](const AdjDesc& a) { return a.colBase; });
+    const AdjDesc& d = regs[ri];
+    const int c = (int)gc - d.colBase;
+    if (c >= d.N) return;
+    double g[3] = {d.floorA * d.adj[13], d.floorA * d.adj[14], d.floorA * d.adj[15]};
+    for (int t = 0; t < d.rowTiles; ++t) {
+        const float* o = d.colpart + ((size_t)t * d.ldx + c) * 3;
+        g[0] += o[0];
+        g[1] += o[1];
+        g[2] += o[2];
+    }
+    d.gX[c] += (float)g[0];
+    d.gX[(size_t)d.N + c] += (float)g[1];
+    d.gX[2 * (size_t)d.N + c] += (float)g[2];
+}
+
+__global__ void adj_output_kernel(const AdjDesc* __restrict__ regs, float* __restrict__ gRt0,
+                                  double* __restrict__ gsig) {
+    const AdjDesc& d = regs[blockIdx.x];
+    if (gRt0 && threadIdx.x < 12) gRt0[(size_t)blockIdx.x * 12 + threadIdx.x] = (float)d.adj[threadIdx.x];
+    if (gsig && threadIdx.x == 12) gsig[blockIdx.x] = d.adj[12];
+}
+
+struct AdjWork {
+    DevBuf xpad, adj, coef, colpart, rowpart, rtmom, desc;
+    std::vector<AdjDesc> regs;
+    long long tiles = 0, rowTiles = 0, cols = 0;
+};
+
+constexpr int kMaxDevices = 64;
+AdjWork* g_adj[kMaxDevices];  // per device, created under that device's lock, never freed
+
+AdjWork* adj_work(const DeviceCtx& ctx) {
+    if (ctx.device < 0 || ctx.device >= kMaxDevices) return nullptr;
+    if (!g_adj[ctx.device]) g_adj[ctx.device] = new AdjWork();
+    return g_adj[ctx.device];
+}
+
+std::vector<float> unrolled_sigmas(float sigma2_0, float factor, int steps) {
+    std::vector<float> s(steps);
+    for (int k = 0; k < steps; ++k) s[k] = (float)((double)sigma2_0 * std::pow((double)factor, k));
+    return s;
+}
+
+size_t ckpt_doubles(const emsa_pair* pairs, int count, int steps, std::vector<size_t>* offs) {
+    size_t tot = 0;
+    for (int k = 0; k < count; ++k) {
+        if (offs) offs->push_back(tot);
+        tot += (size_t)steps * (EMSA_TRACE_W + 5 * (size_t)pairs[k].M);
+    }
+    return tot;
+}
+
+int check(const emsa_pair* pairs, int count, int steps, float sigma2_0, float factor) {
+    if (!pairs || count <= 0 || steps <= 0) return set_error("unrolled: bad sizes"), EMSA_EINVAL;
+    if (!(sigma2_0 > 0.f) || !(factor > 0.f) || !std::isfinite(sigma2_0) || !std::isfinite(factor))
+        return set_error("unrolled: sigma2_0 and factor must be positive"), EMSA_EINVAL;
+    for (int k = 0; k < count; ++k)
+        if (!pairs[k].X || !pairs[k].Y || pairs[k].N <= 0 || pairs[k].M <= 0)
+            return set_error("unrolled: null or empty cloud"), EMSA_EINVAL;
+    return EMSA_OK;
+}
+
+}  // namespace
+
+}  // namespace emsa
+
+using namespace emsa;
+
+extern "C" {
+
+long long emsa_emicp_unrolled_ckpt_size(const emsa_pair* pairs, int count, int steps) {
+    clear_error();
+    if (!pairs || count <= 0 || steps <= 0) return set_error("unrolled: bad sizes"), EMSA_EINVAL;
+    return (long long)ckpt_doubles(pairs, count, steps, nullptr);
+}
+
+int emsa_emicp_unrolled_fwd(const emsa_pair* pairs, int count, const float* dRt0, float sigma2_0,
+                            float factor, float d_02, int steps, float* dRtOut, double* dckpt,
+                            void* stream) {
+    clear_error();
+    int rc;
+    if ((rc = check(pairs, count, steps, sigma2_0, factor)) != EMSA_OK) return rc;
+    if (!dRt0 || !dRtOut || !dckpt) return set_error("unrolled_fwd: null buffer"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<size_t> offs;
+    ckpt_doubles(pairs, count, steps, &offs);
+    std::vector<EmAux> aux(count);
+    std::vector<EmPair> pv(count);
+    for (int k = 0; k < count; ++k) {
+        pv[k] = EmPair{pairs[k].X, pairs[k].N, pairs[k].Y, pairs[k].M};
+        aux[k].trace = dckpt + offs[k];
+        aux[k].ckpt = dckpt + offs[k] + (size_t)steps * EMSA_TRACE_W;
+    }
+    emsa_params p;
+    emsa_default_params(&p);
+    p.d_02 = d_02;
+    rc = em_prepare(*ctx, ctx->em, pv.data(), count, 1, count == 1 ? 0 : default_tile_cols(*ctx),
+                    nullptr, nullptr, s, aux.data());
+    if (rc != EMSA_OK) return rc;
+    return em_enqueue(*ctx, ctx->em, dRt0, dRtOut, nullptr, p,
+                      unrolled_sigmas(sigma2_0, factor, steps), s);
+}
+
+int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0, float sigma2_0,
+                            float factor, float d_02, int steps, const double* dckpt,
+                            const float* dgRt, const emsa_pair_grad* grads, float* dgRt0,
+                            double* dgsigma, void* stream) {
+    clear_error();
+    int rc;
+    if ((rc = check(pairs, count, steps, sigma2_0, factor)) != EMSA_OK) return rc;
+    if (!dRt0 || !dckpt || !dgRt || !grads)
+        return set_error("unrolled_bwd: null buffer"), EMSA_EINVAL;
+    for (int k = 0; k < count; ++k)
+        if (!grads[k].gX || !grads[k].gY)
+            return set_error("unrolled_bwd: null gradient buffer"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    AdjWork* wp = adj_work(*ctx);
+    if (!wp) return set_error("unrolled_bwd: device index out of range"), EMSA_ENODEV;
+    AdjWork& w = *wp;
+    std::vector<size_t> offs;
+    ckpt_doubles(pairs, count, steps, &offs);
+    const int tileCols = 1024;
+    w.regs.assign(count, AdjDesc{});
+    size_t xpadF = 0, colF = 0, rowF = 0, momD = 0;
+    long long tiles = 0, rowTiles = 0, cols = 0;
+    for (int k = 0; k < count; ++k) {
+        AdjDesc& d = w.regs[k];
+        d.X = pairs[k].X;
+        d.Y = pairs[k].Y;
+        d.N = pairs[k].N;
+        d.M = pairs[k].M;
+        d.rec = dckpt + offs[k];
+        d.ck = dckpt + offs[k] + (size_t)steps * EMSA_TRACE_W;
+        d.Rt0 = dRt0 + (size_t)k * 12;
+        d.gX = grads[k].gX;
+        d.gY = grads[k].gY;
+        d.floorA = (double)(1.0f / (float)d.N);
+        d.ldx = (d.N + kColGroup - 1) / kColGroup * kColGroup;
+        d.rowTiles = (d.M + kAdjRows - 1) / kAdjRows;
+        d.waveCols = tileCols / kAdjWaves;
+        d.chunks = (d.ldx + tileCols - 1) / tileCols;
+        d.tileBase = (int)tiles;
+        d.rowBase = (int)rowTiles;
+        d.colBase = (int)cols;
+        tiles += (long long)d.rowTiles * d.chunks;
+        rowTiles += d.rowTiles;
+        cols += d.ldx;
+        d.Xpad = reinterpret_cast<float*>(xpadF);
+        xpadF += 3 * (size_t)d.ldx;
+        d.colpart = reinterpret_cast<float*>(colF);
+        colF += (size_t)d.rowTiles * d.ldx * 3;
+        d.rowpart = reinterpret_cast<float*>(rowF);
+        rowF += (size_t)d.rowTiles * d.chunks * 4 * kAdjRows;
+        d.rtmom = reinterpret_cast<double*>(momD);
+        momD += (size_t)d.rowTiles * kAdjMomW;
+    }
+    if (tiles > 0x7fffffffLL || cols > 0x7fffffffLL)
+        return set_error("unrolled_bwd: batch too large"), EMSA_EINVAL;
+    if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
+    if ((rc = w.colpart.reserve(sizeof(float) * colF)) != EMSA_OK) return rc;
+    if ((rc = w.rowpart.reserve(sizeof(float) * rowF)) != EMSA_OK) return rc;
+    if ((rc = w.rtmom.reserve(sizeof(double) * momD)) != EMSA_OK) return rc;
+    if ((rc = w.adj.reserve(sizeof(double) * kAdjStW * count)) != EMSA_OK) return rc;
+    if ((rc = w.coef.reserve(sizeof(double) * 16 * count)) != EMSA_OK) return rc;
+    if ((rc = w.desc.reserve(sizeof(AdjDesc) * count)) != EMSA_OK) return rc;
+    for (int k = 0; k < count; ++k) {
+        AdjDesc& d = w.regs[k];
+        d.Xpad = w.xpad.as<float>() + reinterpret_cast<size_t>(d.Xpad);
+        d.colpart = w.colpart.as<float>() + reinterpret_cast<size_t>(d.colpart);
+        d.rowpart = w.rowpart.as<float>() + reinterpret_cast<size_t>(d.rowpart);
+        d.rtmom = w.rtmom.as<double>() + reinterpret_cast<size_t>(d.rtmom);
+        d.adj = w.adj.as<double>() + (size_t)k * kAdjStW;
+        d.coef = w.coef.as<double>() + (size_t)k * 16;
+    }
+    EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(AdjDesc) * count,
+                                hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));
+    const AdjDesc* desc = w.desc.as<AdjDesc>();
+    hipLaunchKernelGGL(adj_setup_kernel, dim3(count), dim3(256), 0, s, desc, dgRt);
+    EMSA_HIP_TRY(hipGetLastError());
+    const std::vector<float> sig = unrolled_sigmas(sigma2_0, factor, steps);
+    for (int k = steps - 1; k >= 0; --k) {
+        const double s2 = (double)sig[k];
+        const float negk = (float)(-kLog2e / (2.0 * s2));
+        const float minv = (float)(-1.0 / s2);
+        const double out = (double)expf(-d_02 / sig[k]);
+        const double outlier_dsig = out * (double)d_02 / (s2 * s2);  // d exp(-d02/s2) / d s2
+        const double inv2s4 = 1.0 / (2.0 * s2 * s2);
+        const double fk = std::pow((double)factor, k);
+        hipLaunchKernelGGL(adj_solve_kernel, dim3(count), dim3(64), 0, s, desc, k);
+        hipLaunchKernelGGL(adj_pairs_kernel, dim3((unsigned)tiles), dim3(kWave * kAdjWaves), 0, s,
+                           desc, count, k, negk, minv);
+        hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)rowTiles), dim3(kAdjRows), 0, s, desc,
+                           count, k, inv2s4, outlier_dsig);
+        hipLaunchKernelGGL(adj_finish_kernel, dim3(count), dim3(64), 0, s, desc, k, fk);
+        hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
+                           desc, count, cols);
+        EMSA_HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(adj_output_kernel, dim3(count), dim3(64), 0, s, desc, dgRt0, dgsigma);
+    EMSA_HIP_TRY(hipGetLastError());
+    return EMSA_OK;
+}
+
+}  // extern "C"

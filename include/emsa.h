@@ -161,6 +161,39 @@ int emsa_set_tiling(int blocks_per_cu, int tile_cols);
  * workspace. */
 int emsa_graph_cache_clear(void);
 
+/* ---- Differentiable unrolled EM-ICP (no counterpart in the reference) ----
+ * For callers that train through the registration: `steps` iterations of the
+ * EM-ICP step of src/emicp.cu:443-668 with sigma_k^2 = sigma2_0 * factor^k
+ * (computed in double, rounded to float), outlier term exp(-d_02 / sigma_k^2).
+ * All buffers are device buffers; every pair is one registration from one
+ * initial transform. Asynchronous on `stream`. */
+
+/* Number of doubles of the checkpoint buffer the forward fills: per pair and
+ * step one EMSA_TRACE_W record and (C_r, lambda_r, lambda_r X'_r) per source
+ * row. No affinity is stored; the backward recomputes them. */
+long long emsa_emicp_unrolled_ckpt_size(const emsa_pair *pairs, int count, int steps);
+
+/* Forward. dRt0, dRt_out: [count][12] (R row-major, t); dckpt: the size above. */
+int emsa_emicp_unrolled_fwd(const emsa_pair *pairs, int count, const float *dRt0,
+                            float sigma2_0, float factor, float d_02, int steps, float *dRt_out,
+                            double *dckpt, void *stream);
+
+/* Gradient buffers of one pair: dL/dX [3][N] and dL/dY [3][M], SoA, overwritten. */
+typedef struct emsa_pair_grad {
+    float *gX;
+    float *gY;
+} emsa_pair_grad;
+
+/* Backward (the hand-written adjoint of the forward above, same arguments).
+ * dgRt: [count][12] dL/d(R, t) of the output. Writes grads[k], dgRt0 (nullable,
+ * [count][12] dL/d(R_0, t_0)) and dgsigma2 (nullable, [count] dL/dsigma2_0,
+ * doubles). A step whose Horn solve failed in the forward passes the gradient
+ * of its output through to its input. */
+int emsa_emicp_unrolled_bwd(const emsa_pair *pairs, int count, const float *dRt0,
+                            float sigma2_0, float factor, float d_02, int steps,
+                            const double *dckpt, const float *dgRt, const emsa_pair_grad *grads,
+                            float *dgRt0, double *dgsigma2, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
