@@ -42,7 +42,16 @@ constexpr int kAdjRows = kAdjRPL * kWave;      // 256 rows per tile (one wave co
 constexpr int kAdjWaves = 4;                   // waves per workgroup split the column chunk
 constexpr int kAdjMomW = 16;                   // per-tile partials: dR(9) dt(3) dsig(1) Gfl(3)
 constexpr int kAdjStW = 24;                    // adjoint state: dR(9) dt(3) dsig0(1) Gfl(3) ...
+constexpr int kAdjStageW = 3 * kColGroup;      // staged column values per wave (48)
+constexpr int kAdjStageLd = kWave + 1;         // LDS row stride of the staging tile
+static_assert(kAdjStageW <= kWave, "one summing lane per staged value");
+static_assert(kAdjWaves * kAdjStageW * kAdjStageLd >= kAdjWaves * 4 * kAdjRows,
+              "the row combine reuses the staging memory");
 constexpr float kSentinelA = 1.0e15f;
+// Below this sigma^2 the recomputed affinities subtract the FP32 rounding
+// residual of R y + t (as the forward always does); above it that residual
+// changes e by less than ~1e-5 relative and is left out.
+constexpr float kCompSigma2 = 1e-4f;
 typedef const __attribute__((address_space(4))) float* cfloat_p;
 
 struct AdjDesc {
@@ -62,6 +71,15 @@ struct AdjDesc {
     double floorA;
     int N, M, ldx, rowTiles, chunks, waveCols, tileBase, rowBase, colBase, pad_;
 };
+
+// Orders LDS writes of some lanes before reads by other lanes of the same
+// wave (LDS executes a wave's instructions in order; this keeps the compiler
+// from moving accesses across and waits for outstanding LDS operations).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <class F>
 __device__ inline int find_by(const AdjDesc* __restrict__ d, int n, int b, F key) {
@@ -293,7 +311,11 @@ __device__ inline RowCoef row_coef(const AdjDesc& d, const double* coef, int k, 
 }
 
 // Stage 2: a workgroup = 256 source rows x one column chunk; lanes hold 4
-// rows each, the 4 waves split the chunk.
+// rows each, the 4 waves split the chunk. Per pair: 22 VALU + 1 exp (25 with
+// the compensated difference COMP, used at small sigma) and ~1 more for the
+// column sums. The row coefficients are pre-scaled by -1/sigma^2 so that
+// h = ge e needs one multiply.
+template <bool COMP>
 __global__ void __launch_bounds__(kWave * kAdjWaves)
 adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk, float minv_s2) {
     const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.tileBase; });
@@ -309,8 +331,10 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
     double coef[16];
     for (int i = 0; i < 16; ++i) coef[i] = d.coef[i];
 
+    // per row: p (hi, lo), m * (dB, 2 dC, dW) with m = -1/sigma^2, and dW
     float px[kAdjRPL], py[kAdjRPL], pz[kAdjRPL], lx[kAdjRPL], ly[kAdjRPL], lz[kAdjRPL];
-    float gB[kAdjRPL], gC2[kAdjRPL], gWx[kAdjRPL], gWy[kAdjRPL], gWz[kAdjRPL];
+    float mB[kAdjRPL], mC2[kAdjRPL], mWx[kAdjRPL], mWy[kAdjRPL], mWz[kAdjRPL];
+    float gWx[kAdjRPL], gWy[kAdjRPL], gWz[kAdjRPL];
 #pragma unroll
     for (int j = 0; j < kAdjRPL; ++j) {
         const int r = rt * kAdjRows + j * kWave + lane;
@@ -329,11 +353,16 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
         lx[j] = (float)(pxd - (double)px[j]);
         ly[j] = (float)(pyd - (double)py[j]);
         lz[j] = (float)(pzd - (double)pz[j]);
-        gB[j] = gC2[j] = gWx[j] = gWy[j] = gWz[j] = 0.f;
+        mB[j] = mC2[j] = mWx[j] = mWy[j] = mWz[j] = 0.f;
+        gWx[j] = gWy[j] = gWz[j] = 0.f;
         if (r < M) {
             const RowCoef o = row_coef(d, coef, k, r, yx, yy, yz);
-            gB[j] = (float)o.gB;
-            gC2[j] = (float)(2.0 * o.gC);
+            const double m = (double)minv_s2;
+            mB[j] = (float)(m * o.gB);
+            mC2[j] = (float)(m * 2.0 * o.gC);
+            mWx[j] = (float)(m * o.gW[0]);
+            mWy[j] = (float)(m * o.gW[1]);
+            mWz[j] = (float)(m * o.gW[2]);
             gWx[j] = (float)o.gW[0];
             gWy[j] = (float)o.gW[1];
             gWz[j] = (float)o.gW[2];
@@ -349,6 +378,12 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
 #pragma unroll
     for (int j = 0; j < kAdjRPL; ++j) gpx[j] = gpy[j] = gpz[j] = gs[j] = 0.f;
     float* __restrict__ colpart = d.colpart + ((size_t)rt * ldx) * 3;
+    // Column sums over the wave's 256 rows: each lane adds its 4 rows in
+    // registers, stages (cx, cy, cz) of 16 columns in LDS ([value][lane],
+    // stride 65: conflict-free both ways), then lanes 0..47 each sum one
+    // value over the 64 lanes and store the 48 results contiguously.
+    __shared__ float lds[kAdjWaves * kAdjStageW * kAdjStageLd];
+    float* __restrict__ stage = lds + wave * kAdjStageW * kAdjStageLd;
     for (int c = cb; c < ce; c += kColGroup) {
 #pragma unroll 4
         for (int u = 0; u < kColGroup; ++u) {
@@ -356,57 +391,71 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
             float cx = 0.f, cy = 0.f, cz = 0.f;
 #pragma unroll
             for (int j = 0; j < kAdjRPL; ++j) {
-                const float dx = (xx - px[j]) - lx[j];
-                const float dy = (xy - py[j]) - ly[j];
-                const float dz = (xz - pz[j]) - lz[j];
+                float dx = xx - px[j], dy = xy - py[j], dz = xz - pz[j];
+                if (COMP) {
+                    dx -= lx[j];
+                    dy -= ly[j];
+                    dz -= lz[j];
+                }
                 float s = dx * dx;
                 s = fmaf(dy, dy, s);
                 s = fmaf(dz, dz, s);
                 const float e = __builtin_amdgcn_exp2f(s * negk);
-                const float ge = fmaf(gWz[j], xz, fmaf(gWy[j], xy, fmaf(gWx[j], xx, fmaf(e, gC2[j], gB[j]))));
-                const float t1 = ge * e;
-                const float h = t1 * minv_s2;
+                // h = -(dB + 2 e dC + dW . x) e / sigma^2
+                const float mg = fmaf(mWz[j], xz, fmaf(mWy[j], xy, fmaf(mWx[j], xx, fmaf(e, mC2[j], mB[j]))));
+                const float h = mg * e;
                 gpx[j] = fmaf(h, dx, gpx[j]);
                 gpy[j] = fmaf(h, dy, gpy[j]);
                 gpz[j] = fmaf(h, dz, gpz[j]);
-                gs[j] = fmaf(t1, s, gs[j]);
+                gs[j] = fmaf(h, s, gs[j]);
                 cx = fmaf(h, dx, fmaf(e, gWx[j], cx));
                 cy = fmaf(h, dy, fmaf(e, gWy[j], cy));
                 cz = fmaf(h, dz, fmaf(e, gWz[j], cz));
             }
-            cx = wave_sum(cx);
-            cy = wave_sum(cy);
-            cz = wave_sum(cz);
-            if (lane == 0) {
-                float* o = colpart + (size_t)(c + u) * 3;
-                o[0] = cx;
-                o[1] = cy;
-                o[2] = cz;
-            }
+            stage[(3 * u + 0) * kAdjStageLd + lane] = cx;
+            stage[(3 * u + 1) * kAdjStageLd + lane] = cy;
+            stage[(3 * u + 2) * kAdjStageLd + lane] = cz;
         }
+        wave_lds_sync();
+        if (lane < kAdjStageW) {
+            const float* __restrict__ row = stage + lane * kAdjStageLd;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+            for (int i = 0; i < kWave; i += 4) {
+                a0 += row[i];
+                a1 += row[i + 1];
+                a2 += row[i + 2];
+                a3 += row[i + 3];
+            }
+            colpart[(size_t)c * 3 + lane] = (a0 + a1) + (a2 + a3);
+        }
+        wave_lds_sync();
     }
-    // row partials: combine the 4 waves (fixed order)
-    __shared__ float red[kAdjWaves][4][kAdjRows];
+    // row partials: combine the 4 waves (fixed order), reusing the staging
+    // memory. gs holds sum h s = -(1/sigma^2) sum ge e s; the rows kernel
+    // rescales it.
+    __syncthreads();
+    float* red = lds;  // [kAdjWaves][4][kAdjRows]
 #pragma unroll
     for (int j = 0; j < kAdjRPL; ++j) {
-        red[wave][0][j * kWave + lane] = gpx[j];
-        red[wave][1][j * kWave + lane] = gpy[j];
-        red[wave][2][j * kWave + lane] = gpz[j];
-        red[wave][3][j * kWave + lane] = gs[j];
+        red[(wave * 4 + 0) * kAdjRows + j * kWave + lane] = gpx[j];
+        red[(wave * 4 + 1) * kAdjRows + j * kWave + lane] = gpy[j];
+        red[(wave * 4 + 2) * kAdjRows + j * kWave + lane] = gpz[j];
+        red[(wave * 4 + 3) * kAdjRows + j * kWave + lane] = gs[j];
     }
     __syncthreads();
     float* P = d.rowpart + (size_t)local * 4 * kAdjRows;
     for (int q = 0; q < 4; ++q) {
-        float acc = red[0][q][threadIdx.x];
+        float acc = red[q * kAdjRows + threadIdx.x];
 #pragma unroll
-        for (int w = 1; w < kAdjWaves; ++w) acc += red[w][q][threadIdx.x];
+        for (int w = 1; w < kAdjWaves; ++w) acc += red[(w * 4 + q) * kAdjRows + threadIdx.x];
         P[q * kAdjRows + threadIdx.x] = acc;
     }
 }
 
 // Stage 3: one workgroup per 256-row tile, one thread per row.
 __global__ void __launch_bounds__(kAdjRows)
-adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double inv2s4,
+adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_scale,
                 double outlier_dsig) {
     const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.rowBase; });
     const AdjDesc& d = regs[ri];
@@ -450,7 +499,7 @@ adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double inv2s
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) m[3 * i + j] = gp[i] * y[j];
         for (int i = 0; i < 3; ++i) m[9 + i] = gp[i];
-        m[12] = gsum * inv2s4 + (o.floored ? 0.0 : o.gC * outlier_dsig);
+        m[12] = gsum * gsig_scale + (o.floored ? 0.0 : o.gC * outlier_dsig);
         if (o.floored)
             for (int i = 0; i < 3; ++i) m[13 + i] = o.gV[i];
     }
@@ -676,13 +725,19 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
         const float minv = (float)(-1.0 / s2);
         const double out = (double)expf(-d_02 / sig[k]);
         const double outlier_dsig = out * (double)d_02 / (s2 * s2);  // d exp(-d02/s2) / d s2
-        const double inv2s4 = 1.0 / (2.0 * s2 * s2);
+        // the pairs kernel accumulates sum h s = -(1/sigma^2) sum ge e s, and
+        // dL/dsigma^2 gets sum ge e s / (2 sigma^4) from the affinities
+        const double gsig_scale = -1.0 / (2.0 * s2);
         const double fk = std::pow((double)factor, k);
         hipLaunchKernelGGL(adj_solve_kernel, dim3(count), dim3(64), 0, s, desc, k);
-        hipLaunchKernelGGL(adj_pairs_kernel, dim3((unsigned)tiles), dim3(kWave * kAdjWaves), 0, s,
-                           desc, count, k, negk, minv);
+        if (sig[k] < kCompSigma2)
+            hipLaunchKernelGGL(adj_pairs_kernel<true>, dim3((unsigned)tiles),
+                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, negk, minv);
+        else
+            hipLaunchKernelGGL(adj_pairs_kernel<false>, dim3((unsigned)tiles),
+                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, negk, minv);
         hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)rowTiles), dim3(kAdjRows), 0, s, desc,
-                           count, k, inv2s4, outlier_dsig);
+                           count, k, gsig_scale, outlier_dsig);
         hipLaunchKernelGGL(adj_finish_kernel, dim3(count), dim3(64), 0, s, desc, k, fk);
         hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
                            desc, count, cols);

@@ -19,6 +19,26 @@ __device__ inline float wave_sum(float v) {
     return v;
 }
 
+// Sum over the 64 lanes of a wave with DPP-modified adds only (no LDS
+// permutes); the total is valid in lane 63. Within each 16-lane row a
+// butterfly (quad_perm xor 1, xor 2, row_half_mirror, row_mirror) leaves the
+// row sum in every lane; row_bcast:15 into rows 1 and 3, then row_bcast:31
+// into rows 2 and 3, accumulate the four row sums into row 3.
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK,
+                                                          0xF, false));
+}
+__device__ inline float wave_sum_last(float v) {
+    v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xF>(v);  // row_mirror
+    v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
+    v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
 // Block-wide sum of K doubles per thread for a block of `nwaves` waves.
 // lds must hold nwaves*K doubles. Thread k < K of the block receives total k
 // in out (other threads: unspecified). Fixed reduction order: deterministic.

@@ -89,6 +89,15 @@ def test_adjoint_with_floored_rows(UR):
     _check(UR, X, Y, np.concatenate([Rg.reshape(9), tg]), steps=15, s2=2e-3, d_02=1.0)
 
 
+def test_adjoint_small_sigma_compensated(UR):
+    """sigma^2 from 2e-4 down to 6e-5: the later steps recompute the affinities
+    with the compensated difference (below sigma^2 = 1e-4), the earlier without."""
+    X, Y, Rg, tg = uniform_cube(256, seed=11)
+    Rt_true = np.concatenate([Rg.reshape(9), tg])
+    Rt0 = Rt_true + np.r_[0.002 * np.ones(9), -0.001 * np.ones(3)]  # near the optimum
+    _check(UR, X, Y, Rt_true, steps=12, s2=2e-4, Rt0=Rt0)
+
+
 def test_adjoint_batch_equals_single(UR):
     pairs = [uniform_cube(320, seed=60 + k) for k in range(3)]
     X = torch.tensor(np.stack([p[0] for p in pairs]), device="cuda").requires_grad_(True)
