@@ -33,6 +33,35 @@ VALU_PER_PAIR = 15            # issued per pair: 6 sub (compensated), 3 (d^2), 1
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 
 
+def fwd_bwd_emicp(torch, dev, stream, timed, steps, B=32, n=4096, unrolled=20):
+    from cuda_emicp_softassign_amd import unrolled as UR
+    from cuda_emicp_softassign_amd.clouds import uniform_cube
+    pairs = [uniform_cube(n, seed=4000 + b) for b in range(B)]
+    X = torch.tensor(np.stack([p[0] for p in pairs]), device=dev)
+    Y = torch.tensor(np.stack([p[1] for p in pairs]), device=dev)
+    tgt = torch.tensor(np.stack([np.concatenate([p[2].reshape(9), p[3]]) for p in pairs]),
+                       dtype=torch.float32, device=dev)
+
+    def fwd():
+        with torch.no_grad():
+            UR.emicp_unrolled(X, Y, None, 0.01, steps=unrolled)
+
+    def fwdbwd():
+        Xg = X.detach().requires_grad_(True)
+        Yg = Y.detach().requires_grad_(True)
+        sg = torch.tensor(0.01, dtype=torch.float64, requires_grad=True)
+        Rt = UR.emicp_unrolled(Xg, Yg, None, sg, steps=unrolled)
+        ((Rt - tgt) ** 2).sum().backward()
+
+    fwd()
+    fwdbwd()
+    f_ms, _ = timed(fwd, steps)
+    fb_ms, _ = timed(fwdbwd, steps)
+    return {"workload": "EM-ICP unrolled, %d pairs of N=M=%d, %d sigma^2 steps, "
+                        "grads w.r.t. X, Y, sigma2_0" % (B, n, unrolled),
+            "fwd_ms": f_ms, "fwd_bwd_ms": fb_ms, "ratio": fb_ms / f_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +213,11 @@ def main():
 
     small()
     out["ms_per_registration_n1007"], _ = timed(small, 5)
+
+    # differentiable registration: 32 pairs of N=M=4096, 20 unrolled sigma^2
+    # steps (0.01 * 0.9^k), loss ||R - R_true||_F^2 + ||t - t_true||^2,
+    # gradients w.r.t. X, Y and sigma2_0 through the hand-written adjoint
+    out["fwd_bwd"] = fwd_bwd_emicp(torch, dev, stream, timed, args.steps)
 
     if not args.no_check:
         from oracle import oracle as O
