@@ -83,6 +83,16 @@ SIGNATURES = {
                                           C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,
                                           C.POINTER(emsa_pair_grad), C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
+    "emsa_softassign_unrolled_ckpt_bytes": (C.c_longlong, [C.c_int] * 6),
+    "emsa_softassign_unrolled_fwd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                               C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                               C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]),
+    "emsa_softassign_unrolled_bwd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                               C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                               C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

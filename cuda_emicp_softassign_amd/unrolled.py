@@ -126,3 +126,89 @@ def emicp_unrolled(X, Y, Rt0=None, sigma2_0=0.01, factor=0.9, steps=20, d_02=0.0
 def split_rt(Rt):
     """(..., 12) -> R (..., 3, 3), t (..., 3)."""
     return Rt[..., :9].reshape(Rt.shape[:-1] + (3, 3)), Rt[..., 9:]
+
+
+# ---------------------------------------------------------------------------
+# Softassign
+# ---------------------------------------------------------------------------
+
+class SoftassignUnrolled:
+    """Namespace for the autograd.Function of the unrolled Softassign."""
+    fn = None
+
+
+def _sa_function():
+    if SoftassignUnrolled.fn is not None:
+        return SoftassignUnrolled.fn
+    import torch
+
+    class _F(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, X, Y, Rt0, T_0, TFACTOR, alpha, moutlier, JMAX, I0, I1):
+            _check_inputs(X, Y, Rt0)
+            L = _lib.load()
+            B, N, M = X.shape[0], X.shape[1], Y.shape[1]
+            Xs = X.detach().transpose(1, 2).contiguous()
+            Ys = Y.detach().transpose(1, 2).contiguous()
+            R0 = Rt0.detach().contiguous()
+            T0 = float(T_0)
+            mo = -1.0 if moutlier is None else float(moutlier)
+            nb = _lib.check("softassign_unrolled_ckpt_bytes", L.emsa_softassign_unrolled_ckpt_bytes(
+                B, N, M, JMAX, I0, I1))
+            ckpt = torch.empty((nb + 7) // 8, dtype=torch.float64, device=X.device)
+            out = torch.empty(B, 12, dtype=torch.float32, device=X.device)
+            stream = torch.cuda.current_stream(X.device).cuda_stream
+            _lib.check("softassign_unrolled_fwd", L.emsa_softassign_unrolled_fwd(
+                C.c_void_p(Xs.data_ptr()), C.c_void_p(Ys.data_ptr()), B, N, M,
+                C.c_void_p(R0.data_ptr()), T0, float(TFACTOR), float(alpha), mo, JMAX, I0, I1,
+                C.c_void_p(out.data_ptr()), C.c_void_p(ckpt.data_ptr()), C.c_void_p(stream)))
+            ctx.save_for_backward(Xs, Ys, R0, ckpt)
+            ctx.args = (T0, float(TFACTOR), float(alpha), mo, int(JMAX), int(I0), int(I1))
+            ctx.T_like = T_0 if torch.is_tensor(T_0) else None
+            return out
+
+        @staticmethod
+        def backward(ctx, gout):
+            Xs, Ys, R0, ckpt = ctx.saved_tensors
+            T0, TF, alpha, mo, JMAX, I0, I1 = ctx.args
+            B, N, M = Xs.shape[0], Xs.shape[2], Ys.shape[2]
+            g = gout.detach().to(torch.float32).contiguous()
+            gXs = torch.empty_like(Xs)
+            gYs = torch.empty_like(Ys)
+            gR0 = torch.empty_like(R0)
+            gT = torch.empty(B, dtype=torch.float64, device=Xs.device)
+            stream = torch.cuda.current_stream(Xs.device).cuda_stream
+            _lib.check("softassign_unrolled_bwd", _lib.load().emsa_softassign_unrolled_bwd(
+                C.c_void_p(Xs.data_ptr()), C.c_void_p(Ys.data_ptr()), B, N, M,
+                C.c_void_p(R0.data_ptr()), T0, TF, alpha, mo, JMAX, I0, I1,
+                C.c_void_p(ckpt.data_ptr()), C.c_void_p(g.data_ptr()),
+                C.c_void_p(gXs.data_ptr()), C.c_void_p(gYs.data_ptr()),
+                C.c_void_p(gR0.data_ptr()), C.c_void_p(gT.data_ptr()), C.c_void_p(stream)))
+            gt0 = None
+            if ctx.T_like is not None:
+                gt0 = gT.sum().to(dtype=ctx.T_like.dtype, device=ctx.T_like.device)
+            return (gXs.transpose(1, 2), gYs.transpose(1, 2), gR0, gt0,
+                    None, None, None, None, None, None)
+
+    SoftassignUnrolled.fn = _F
+    return _F
+
+
+def softassign_unrolled(X, Y, Rt0=None, T_0=0.5, TFACTOR=0.9, alpha=0.05, moutlier=None,
+                        JMAX=20, I0=1, I1=3):
+    """Unrolled Softassign (JMAX temperatures x I0 inner iterations x I1
+    Sinkhorn sweeps), differentiable with respect to X (B, N, 3), Y (B, M, 3),
+    Rt0 (B, 12) and T_0 (a float or a 0-dim tensor; its gradient is the sum
+    over the batch). moutlier None: exp(-1) / sqrt(T_0). Returns Rt (B, 12)."""
+    import torch
+    squeeze = X.dim() == 2
+    if squeeze:
+        X, Y = X.unsqueeze(0), Y.unsqueeze(0)
+        if Rt0 is not None and Rt0.dim() == 1:
+            Rt0 = Rt0.unsqueeze(0)
+    if Rt0 is None:
+        eye = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float32,
+                           device=X.device)
+        Rt0 = eye.expand(X.shape[0], 12).contiguous()
+    out = _sa_function().apply(X, Y, Rt0, T_0, TFACTOR, alpha, moutlier, JMAX, I0, I1)
+    return out[0] if squeeze else out

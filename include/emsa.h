@@ -194,6 +194,34 @@ int emsa_emicp_unrolled_bwd(const emsa_pair *pairs, int count, const float *dRt0
                             const double *dckpt, const float *dgRt, const emsa_pair_grad *grads,
                             float *dgRt0, double *dgsigma2, void *stream);
 
+/* ---- Differentiable unrolled Softassign (no counterpart in the reference) --
+ * For callers that train through the registration: JMAX temperatures
+ * (T <- T * TFACTOR in float, as src/softassign.cu:820) of I0 inner iterations
+ * (src/softassign.cu:494-818) with I1 (1..4) Sinkhorn sweeps each, for B
+ * registrations of equal sizes at once. dX: [B][3][N], dY: [B][3][M] (SoA per
+ * registration), dRt0, dRt_out: [B][12]; moutlier < 0 selects the default
+ * exp(-1) / sqrt(T_0) (and its T_0 gradient). All buffers are device buffers;
+ * asynchronous on `stream`. */
+
+/* Bytes of the checkpoint buffer the forward fills (per registration and
+ * inner iteration: a 40-double record, per sweep the row/column sums and
+ * masses, the slack vectors at the iteration start, s_r and D'_r). */
+long long emsa_softassign_unrolled_ckpt_bytes(int B, int N, int M, int JMAX, int I0, int I1);
+
+int emsa_softassign_unrolled_fwd(const float *dX, const float *dY, int B, int N, int M,
+                                 const float *dRt0, float T_0, float TFACTOR, float alpha,
+                                 float moutlier, int JMAX, int I0, int I1, float *dRt_out,
+                                 void *dckpt, void *stream);
+
+/* Backward (hand-written adjoint, same arguments as the forward). dgRt:
+ * [B][12] dL/d(R, t) of the output. Writes dgX [B][3][N], dgY [B][3][M],
+ * dgRt0 (nullable, [B][12]) and dgT0 (nullable, [B] doubles, dL/dT_0). */
+int emsa_softassign_unrolled_bwd(const float *dX, const float *dY, int B, int N, int M,
+                                 const float *dRt0, float T_0, float TFACTOR, float alpha,
+                                 float moutlier, int JMAX, int I0, int I1, const void *dckpt,
+                                 const float *dgRt, float *dgX, float *dgY, float *dgRt0,
+                                 double *dgT0, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,3 +75,54 @@ def unrolled(X, Y, Rt0, sigma2_0, factor, steps, d_02=0.01):
     for k in range(steps):
         R, t = emicp_step(X, Y, R, t, sigma2_0 * factor ** k, d_02)
     return torch.cat([R.reshape(9), t])
+
+
+# ---------------------------------------------------------------------------
+# Softassign: one inner iteration of src/softassign.cu:494-818 (== the C
+# oracle's or_softassign_inner), rows r over the target X (N), columns c over
+# the source Y (M), with the slack vectors o_row, o_col carried across the
+# whole run (src/softassign.cu:385-390).
+# ---------------------------------------------------------------------------
+
+def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col):
+    """Returns (R', t', o_row', o_col', record) with record =
+    dict(Sigma, Xc, Yc, S) (S[a][b] = sum (source)_a (target)_b)."""
+    P = Y @ R.T + t
+    D2 = ((X[:, None, :] - P[None, :, :]) ** 2).sum(-1)  # (N, M)
+    Mt = torch.exp(-(D2 - alpha) / T) / torch.sqrt(T)    # updateM
+    b = None
+    for _ in range(I1):
+        a = Mt.sum(1) + o_row
+        Mt = Mt / a[:, None]
+        o_row = o_row / a
+        b = Mt.sum(0) + o_col
+        Mt = Mt / b[None, :]
+        o_col = o_col / b
+    s = Mt.sum(1)
+    Sg = s.abs().sum()
+    Xt = s[:, None] * X
+    Yt = b[:, None] * Y
+    Xc = Xt.sum(0) / Sg
+    Yc = Yt.sum(0) / Sg
+    Xh = Xt - Xc
+    Yh = Yt - Yc
+    Dm = Mt @ Yh                  # (N, 3)
+    S = Dm.T @ Xh                 # S[a][b]: a over the source (D), b over the target
+    Rn, tn = horn(Xc, Yc, S)
+    return Rn, tn, o_row, o_col, dict(Sigma=Sg, Xc=Xc, Yc=Yc, S=S)
+
+
+def softassign_unrolled(X, Y, Rt0, T_0, JMAX, I0, I1, TFACTOR, alpha, moutlier=None):
+    """JMAX temperatures T_j = T_0 * TFACTOR^j, I0 inner iterations each;
+    moutlier None: exp(-1) / sqrt(T_0) (differentiable in T_0). Returns Rt (12,)."""
+    R = Rt0[:9].reshape(3, 3)
+    t = Rt0[9:]
+    mo = torch.exp(torch.tensor(-1.0, dtype=X.dtype)) / torch.sqrt(T_0) if moutlier is None \
+        else torch.tensor(moutlier, dtype=X.dtype)
+    o_row = mo * torch.ones(X.shape[0], dtype=X.dtype)
+    o_col = mo * torch.ones(Y.shape[0], dtype=X.dtype)
+    for j in range(JMAX):
+        T = T_0 * TFACTOR ** j
+        for _ in range(I0):
+            R, t, o_row, o_col, _ = softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col)
+    return torch.cat([R.reshape(9), t])
