@@ -12,6 +12,7 @@ namespace emsa {
 namespace {
 
 constexpr size_t kMaxGraphs = 32;
+constexpr size_t kMaxSeen = 64;
 constexpr int kDefaultTileCols = 2048;
 uint64_t g_clock = 0;
 
@@ -42,10 +43,12 @@ void drop_entry(GraphEntry& e) {
 
 int tile_cols(const DeviceCtx& ctx) { return default_tile_cols(ctx); }
 
-// Finds or builds the captured launch sequence for this call and launches it
-// on `user`. The sequence is recorded on the library's own stream (the null
-// stream cannot be captured) and only kernels are recorded: the workspace
-// is set up (tickets zeroed, descriptors uploaded) before capture.
+// Runs this call's launch sequence on `user`: replayed from the cache when
+// the call was recorded before; launched directly (with the context's own
+// workspace) the first time a call is seen; recorded and replayed when it is
+// seen again. A sequence is recorded on the library's own stream (the null
+// stream cannot be captured) and only kernels are recorded: the workspace is
+// set up (descriptors uploaded) before capture.
 int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pairs, int count,
               int n_starts, int tileCols, const float* dRt0, float* dRtOut, double* dResid,
               const emsa_params& p, const std::vector<float>& sig, hipStream_t user) {
@@ -57,6 +60,31 @@ int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pa
             return EMSA_OK;
         }
     }
+    auto seen = std::find(ctx.seen.begin(), ctx.seen.end(), key);
+    if (seen == ctx.seen.end()) {
+        if (ctx.seen.size() >= kMaxSeen) ctx.seen.erase(ctx.seen.begin());
+        ctx.seen.push_back(key);
+        // direct launch with the next ring workspace: waits only for the call
+        // that used this slot kRing calls ago (its descriptors and host table
+        // are then free to rewrite)
+        const int slot = ctx.ringNext;
+        ctx.ringNext = (slot + 1) % DeviceCtx::kRing;
+        EmWork& w = ctx.ring[slot];
+        if (!ctx.ringDone[slot])
+            EMSA_HIP_TRY(hipEventCreateWithFlags(&ctx.ringDone[slot], hipEventDisableTiming));
+        if (ctx.ringPending[slot]) {
+            EMSA_HIP_TRY(hipEventSynchronize(ctx.ringDone[slot]));
+            ctx.ringPending[slot] = false;
+        }
+        int rc = em_prepare(ctx, w, pairs, count, n_starts, tileCols, nullptr, nullptr, user,
+                            nullptr, false);
+        if (rc != EMSA_OK) return rc;
+        if ((rc = em_enqueue(ctx, w, dRt0, dRtOut, dResid, p, sig, user)) != EMSA_OK) return rc;
+        EMSA_HIP_TRY(hipEventRecord(ctx.ringDone[slot], user));
+        ctx.ringPending[slot] = true;
+        return EMSA_OK;
+    }
+    ctx.seen.erase(seen);
     if (ctx.graphs.size() >= kMaxGraphs) {  // evict the least recently used
         EMSA_HIP_TRY(hipDeviceSynchronize());
         auto it = std::min_element(ctx.graphs.begin(), ctx.graphs.end(),
@@ -313,6 +341,7 @@ int emsa_graph_cache_clear(void) {
     EMSA_HIP_TRY(hipDeviceSynchronize());
     for (auto& e : ctx->graphs) drop_entry(*e);
     ctx->graphs.clear();
+    ctx->seen.clear();
     return EMSA_OK;
 }
 

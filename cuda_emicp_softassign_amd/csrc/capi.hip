@@ -100,6 +100,10 @@ DeviceCtx* current_ctx() {
             set_error("hipStreamCreateWithFlags failed");
             return nullptr;
         }
+        if (hipEventCreateWithFlags(&c->emDone, hipEventDisableTiming) != hipSuccess) {
+            set_error("hipEventCreateWithFlags failed");
+            return nullptr;
+        }
         g_ctx[dev] = std::move(c);
     }
     return g_ctx[dev].get();

@@ -396,8 +396,13 @@ std::vector<float> sigma_schedule(const emsa_params& p) {
 }
 
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
-               int tileCols, double* trace, double* rows, hipStream_t s, const EmAux* aux) {
+               int tileCols, double* trace, double* rows, hipStream_t s, const EmAux* aux,
+               bool sync_upload) {
     if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
+    if (&w == &ctx.em && ctx.emPending) {  // a sequence on another stream may still read it
+        EMSA_HIP_TRY(hipEventSynchronize(ctx.emDone));
+        ctx.emPending = false;
+    }
     w.regs.clear();
     size_t xpadF = 0, partF = 0, momD = 0;
     long long tiles = 0, rowBlocks = 0;
@@ -470,13 +475,12 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     }
     EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(EmRegDesc) * nregs,
                                 hipMemcpyHostToDevice, s));
-    EMSA_HIP_TRY(hipStreamSynchronize(s));  // w.regs may be rewritten by the next prepare
+    if (sync_upload) EMSA_HIP_TRY(hipStreamSynchronize(s));  // w.regs may be rewritten next
     return EMSA_OK;
 }
 
 int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
                const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s) {
-    (void)ctx;
     const int nregs = (int)w.regs.size();
     const EmRegDesc* desc = w.desc.as<EmRegDesc>();
     hipLaunchKernelGGL(emicp_setup_kernel, dim3(nregs), dim3(256), 0, s, desc, dRt0);
@@ -499,6 +503,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     }
     hipLaunchKernelGGL(emicp_output_kernel, dim3(nregs), dim3(64), 0, s, desc, dRtOut, dResid);
     EMSA_HIP_TRY(hipGetLastError());
+    if (&w == &ctx.em) {
+        EMSA_HIP_TRY(hipEventRecord(ctx.emDone, s));
+        ctx.emPending = true;
+    }
     return EMSA_OK;
 }
 

@@ -115,8 +115,22 @@ struct DeviceCtx {
     // softassign workspaces
     DevBuf K, u, v, orow, ocol, colpart, samom, sastate;
     // captured EM-ICP launch sequences of the device entry points, replayed
-    // when a call repeats the same buffers, sizes and parameters
+    // when a call repeats the same buffers, sizes and parameters; keys seen
+    // once (a sequence is recorded on its second call)
     std::vector<std::unique_ptr<GraphEntry>> graphs;
+    std::vector<std::vector<uint64_t>> seen;
+    // workspaces of calls launched directly (not yet recorded), used in turn
+    // so that consecutive calls do not wait for each other; ringDone[i] marks
+    // the completion of the last sequence that used ring[i]
+    static constexpr int kRing = 4;
+    EmWork ring[kRing];
+    hipEvent_t ringDone[kRing] = {};
+    bool ringPending[kRing] = {};
+    int ringNext = 0;
+    // completion of the last launch sequence that used `em` on any stream;
+    // em_prepare waits for it before rewriting that workspace
+    hipEvent_t emDone = nullptr;
+    bool emPending = false;
     int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
     int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)
 };
@@ -140,9 +154,11 @@ struct EmAux {
     double* trace;
     double* ckpt;
 };
+// sync_upload: wait for the descriptor upload before returning (callers that
+// rewrite w.regs before the stream has consumed it need this).
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
                int tileCols, double* trace, double* rows, hipStream_t s,
-               const EmAux* aux = nullptr);
+               const EmAux* aux = nullptr, bool sync_upload = true);
 
 // Enqueues setup, one launch per sigma^2 step, and the output kernel.
 // dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.

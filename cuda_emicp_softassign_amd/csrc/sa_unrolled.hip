@@ -40,7 +40,7 @@ namespace emsa {
 
 namespace {
 
-constexpr int kRW = 2;                 // rows per wave in row passes
+constexpr int kRW = 4;                 // rows per wave in row passes
 constexpr int kWG = 4;                 // waves per workgroup
 constexpr int kRowsPB = kRW * kWG;     // rows per row-pass workgroup
 constexpr int kColsPB = 256;           // columns per column-pass workgroup
@@ -71,6 +71,7 @@ struct Sau {
     float* orow;       // [B][N]
     float* ocol;       // [B][ldk]
     float* bcol;       // [B][ldk]
+    float* wfin;       // [B][4][ldk] final-pass column weights V, V B y
     float* cpart;      // [B][H][4][ldk]
     double* rmom;      // [B][rowWaves][kMomW]
     double* cby;       // [B][colBlocks][3]
@@ -215,7 +216,7 @@ __global__ void __launch_bounds__(kWave * kWG)
 sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
     constexpr int NW = (MODE == kRowFinal) ? 4 : 1;
     const int b = blockIdx.y;
-    const int N = d.N, M = d.M, ldk = d.ldk;
+    const int N = d.N, ldk = d.ldk;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int gw = blockIdx.x * kWG + wave;
@@ -223,8 +224,7 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
     const float* X = d.X + (size_t)b * 3 * N;
     const float* P = d.P + (size_t)b * 3 * ldk;
     const float* vb = d.v + (size_t)b * ldk;
-    const float* bb = d.bcol + (size_t)b * ldk;
-    const float* Yb = d.Y + (size_t)b * 3 * M;
+    const float* wfb = d.wfin + (size_t)b * 4 * ldk;
     const float* zcb = d.zc + (size_t)b * ldk;
     const bool ones = (MODE == kRowFwd && k == 1);  // v^0 = 1
 
@@ -263,18 +263,12 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) w[i][0] = zz[i];
         } else {
-            float vv[4], bc[4];
-            load4(vb + c, vv);
-            load4(bb + c, bc);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int cc = min(c + i, M - 1);
-                const bool in = (c + i) < M;
-                const float base = vv[i] * bc[i];
-                w[i][0] = vv[i];
-                w[i][1 % NW] = in ? base * Yb[cc] : 0.f;
-                w[i][2 % NW] = in ? base * Yb[(size_t)M + cc] : 0.f;
-                w[i][3 % NW] = in ? base * Yb[2 * (size_t)M + cc] : 0.f;
+            for (int q = 0; q < NW; ++q) {
+                float ww[4];
+                load4(wfb + (size_t)q * ldk + c, ww);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) w[i][q] = ww[i];
             }
         }
 #pragma unroll
@@ -513,12 +507,25 @@ __global__ void __launch_bounds__(kWave * kWG) sau_colfin_kernel(Sau d, int k, i
             d.bcol[(size_t)b * ldk + c] = Bc;
             ck[ck_B(d, k) + c] = Bc;
             ck[ck_cm(d, k) + c] = mass;
-            by[0] = (double)Bc * Yb[c];
-            by[1] = (double)Bc * Yb[(size_t)M + c];
-            by[2] = (double)Bc * Yb[2 * (size_t)M + c];
+            const float y0 = Yb[c], y1 = Yb[(size_t)M + c], y2 = Yb[2 * (size_t)M + c];
+            by[0] = (double)Bc * y0;
+            by[1] = (double)Bc * y1;
+            by[2] = (double)Bc * y2;
+            if (k == d.I1) {  // weights of the final row pass: V and V B y
+                float* wf = d.wfin + (size_t)b * 4 * ldk;
+                const float V = vb[c], base = V * Bc;
+                wf[c] = V;
+                wf[(size_t)ldk + c] = base * y0;
+                wf[2 * (size_t)ldk + c] = base * y1;
+                wf[3 * (size_t)ldk + c] = base * y2;
+            }
         } else {
             vb[c] = 0.f;
             d.bcol[(size_t)b * ldk + c] = 0.f;
+            if (k == d.I1) {
+                float* wf = d.wfin + (size_t)b * 4 * ldk;
+                for (int q = 0; q < 4; ++q) wf[(size_t)q * ldk + c] = 0.f;
+            }
         }
         __shared__ double lds[kWG * 3];
         double total;
@@ -1055,7 +1062,7 @@ sau_adj_output_kernel(Sau d, float* __restrict__ gRt0, double* __restrict__ gT0,
 // Host side
 // ---------------------------------------------------------------------------
 struct SauWork {
-    DevBuf P, state, u, v, orow, ocol, bcol, cpart, rmom, cby;
+    DevBuf P, state, u, v, orow, ocol, bcol, wfin, cpart, rmom, cby;
     DevBuf gU, gV, gOr, gOc, L, Q, zr, zc, adj, fcol, frow, fmomr, fmomc;
 };
 constexpr int kMaxDevices = 64;
@@ -1141,6 +1148,7 @@ int sau_bind_forward(Sau& d, SauWork& w, const float* X, const float* Y, const f
     if ((rc = w.orow.reserve(sizeof(float) * B * N)) != EMSA_OK) return rc;
     if ((rc = w.ocol.reserve(sizeof(float) * B * ldk)) != EMSA_OK) return rc;
     if ((rc = w.bcol.reserve(sizeof(float) * B * ldk)) != EMSA_OK) return rc;
+    if ((rc = w.wfin.reserve(sizeof(float) * B * 4 * ldk)) != EMSA_OK) return rc;
     if ((rc = w.cpart.reserve(sizeof(float) * B * d.H * 4 * ldk)) != EMSA_OK) return rc;
     if ((rc = w.rmom.reserve(sizeof(double) * B * d.rowWaves * kMomW)) != EMSA_OK) return rc;
     if ((rc = w.cby.reserve(sizeof(double) * B * d.colBlocks * 3)) != EMSA_OK) return rc;
@@ -1154,6 +1162,7 @@ int sau_bind_forward(Sau& d, SauWork& w, const float* X, const float* Y, const f
     d.orow = w.orow.as<float>();
     d.ocol = w.ocol.as<float>();
     d.bcol = w.bcol.as<float>();
+    d.wfin = w.wfin.as<float>();
     d.cpart = w.cpart.as<float>();
     d.rmom = w.rmom.as<double>();
     d.cby = w.cby.as<double>();

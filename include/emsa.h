@@ -135,9 +135,10 @@ int emsa_emicp_batched(const emsa_pair *pairs, int count, const float *Rt0, int 
                        const emsa_params *p, float *Rt_out, double *residual);
 
 /* Device buffers (pairs point to device SoA arrays; dRt0, dRt_out, dResidual
- * are device arrays). The launch sequence is captured once in a hipGraph and
- * replayed on `stream` whenever a call repeats the same buffers, sizes and
- * parameters. Asynchronous; errors detected at launch are returned. */
+ * are device arrays). The first call with given buffers, sizes and parameters
+ * launches its kernels directly; when the same call comes again its launch
+ * sequence is recorded in a hipGraph and replayed on `stream` from then on.
+ * Asynchronous; errors detected at launch are returned. */
 int emsa_emicp_batched_dev(const emsa_pair *pairs, int count, const float *dRt0, int n_starts,
                            const emsa_params *p, float *dRt_out, double *dResidual,
                            void *stream);
