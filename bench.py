@@ -31,6 +31,49 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 1 (B), 6 (W fma)
 VALU_PER_PAIR = 15            # issued per pair: 6 sub (compensated), 3 (d^2), 1 scale, 5 accumulate
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+# HBM bytes per launch of the batched affinity kernel on the default workload
+# (8 x 10k, 8e8 pairs): rocprofv3 --pmc FETCH_SIZE (x2, from a 256 MiB copy
+# calibration in the same process) + WRITE_SIZE, run on `bench.py --counters
+# affinity`; 19.7 MiB fetched + 7.7 MiB written
+AFFINITY_TRAFFIC_BYTES = 28.7e6
+
+
+def counters_mode(which, E, _lib, torch):
+    """Workloads for rocprofv3 --pmc runs (no hipGraph in the process):
+    affinity -- 20 launches of the batched EM-ICP affinity kernel on the 8 x 10k
+    workload (8e8 pairs each); softassign -- one materialised N = M = 5032
+    registration at the reference defaults. Then a 256 MiB device copy of known
+    size for the counter scale."""
+    from cuda_emicp_softassign_amd.clouds import surface_with_outliers, uniform_cube
+    dev = torch.device("cuda:0")
+    out = {"counters": which}
+    if which == "affinity":
+        keep = []
+        arr = (_lib.emsa_pair * 8)()
+        for k in range(8):
+            X, Y, _, _ = uniform_cube(10000, seed=10 + k)
+            dx = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+            dy = torch.from_numpy(np.ascontiguousarray(Y.T)).to(dev)
+            keep += [dx, dy]
+            arr[k] = _lib.emsa_pair(dx.data_ptr(), 10000, dy.data_ptr(), 10000)
+        us, npairs = ctypes.c_double(), ctypes.c_double()
+        _lib.check("probe_affinity", _lib.load().emsa_probe_emicp_affinity(
+            arr, 8, 1, ctypes.c_float(1e-3), 20, ctypes.byref(us), ctypes.byref(npairs)))
+        out.update(avg_launch_us=us.value, pairs_per_launch=npairs.value)
+    else:
+        X, Y, _, _ = surface_with_outliers(5032, seed=1)
+        p = E.registrationParameters()
+        t0 = time.perf_counter()
+        E.softassign(X, Y, np.eye(3), np.zeros(3), p, variant="materialised")
+        out["ms"] = (time.perf_counter() - t0) * 1e3
+    src = torch.ones(256 * (1 << 18), device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_bytes_read"] = src.numel() * 4
+    out["copy_bytes_written"] = src.numel() * 4
+    print(json.dumps(out), flush=True)
 
 
 def config3(E, torch, dev, sptr, timed, steps, p, n_inst=256):
@@ -90,6 +133,34 @@ def config3(E, torch, dev, sptr, timed, steps, p, n_inst=256):
             "batched_ms": ms_b, "plain_ms": ms_p, "tile_cols": tile,
             "pairs_per_s_batched": pairs_per_step * n_sigma * 1000.0 / ms_b,
             "rot_err_batched_vs_plain_rad": agree}
+
+
+def softassign_plain(E, n=5032):
+    """Softassign at the reference defaults (JMAX 100, I0 5, I1 3, alpha 3,
+    T_0 100, TFACTOR 0.95: 500 inner iterations), N = M = 5032 surface pairs
+    with 10% outliers, both variants, one registration each after a warm-up.
+    Algorithmic work per inner iteration: materialised 4 (1 + 2 I1) = 28 bytes
+    per matrix element; log-domain 2 I1 + 1 = 7 exp per element."""
+    from cuda_emicp_softassign_amd.clouds import surface_with_outliers
+    X, Y, _, _ = surface_with_outliers(n, seed=1)
+    p = E.registrationParameters()
+    inner = p.JMAX * p.I0
+    elems = float(n) * n
+    out = {"workload": "Softassign N=M=%d, JMAX=%d, I0=%d, I1=%d (%d inner iterations)"
+                       % (n, p.JMAX, p.I0, p.I1, inner)}
+    for variant in ("materialised", "logdomain"):
+        E.softassign(X, Y, np.eye(3), np.zeros(3), p, variant=variant)
+        t0 = time.perf_counter()
+        E.softassign(X, Y, np.eye(3), np.zeros(3), p, variant=variant)
+        ms = (time.perf_counter() - t0) * 1e3
+        r = {"ms_per_registration": ms, "inner_iterations_per_s": inner * 1e3 / ms}
+        if variant == "materialised":
+            r["algorithmic_GB_per_s"] = 4.0 * (1 + 2 * p.I1) * elems * inner / (ms * 1e-3) / 1e9
+            r["frac_of_hbm_peak"] = r["algorithmic_GB_per_s"] / HBM_PEAK_GBS
+        else:
+            r["algorithmic_exp_per_s"] = (2 * p.I1 + 1) * elems * inner / (ms * 1e-3)
+        out[variant] = r
+    return out
 
 
 def fwd_bwd_softassign(torch, dev, timed, steps, B=32, n=4096, JMAX=20):
@@ -172,6 +243,13 @@ def main():
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the 256-instance x 24-start batched workload")
     ap.add_argument("--no-grad", action="store_true", help="skip the fwd+bwd workloads")
+    ap.add_argument("--no-softassign", action="store_true",
+                    help="skip the N=M=5032 Softassign timing")
+    ap.add_argument("--counters", choices=["affinity", "softassign"], default=None,
+                    help="counter-collection mode: run only this kernel's workload with direct "
+                         "launches (no graph replay), then a 256 MiB device copy that "
+                         "calibrates FETCH_SIZE / WRITE_SIZE; prints a short JSON line")
+
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path",
@@ -183,6 +261,10 @@ def main():
     import cuda_emicp_softassign_amd as E
     from cuda_emicp_softassign_amd import _lib
     from cuda_emicp_softassign_amd.clouds import geodesic, uniform_cube
+
+    if args.counters:
+        counters_mode(args.counters, E, _lib, torch)
+        return
 
     n, B = args.n, args.pairs
     p = E.registrationParameters()
@@ -290,7 +372,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": AFFINITY_TRAFFIC_BYTES if (n, B) == (10000, 8) else None,
             "avg_launch_us": us.value,
             "pairs_per_launch": npairs.value,
             "algorithmic": "%d FP32 flops + 1 exp per (x, y) pair" % FLOPS_PER_PAIR,
@@ -326,6 +408,9 @@ def main():
                           "softassign": fwd_bwd_softassign(torch, dev, timed, args.steps)}
     if not args.no_config3:
         out["config3"] = config3(E, torch, dev, sptr, timed, args.steps, p)
+    if not args.no_softassign:
+        out["softassign"] = softassign_plain(E)
+
 
     if not args.no_cpu:
         # CPU leg: the oracle (test infrastructure) runs on the host cores --

@@ -78,9 +78,10 @@ struct SaBufs {
     float* orow;      // [N]
     float* ocol;      // [M]
     float* bcol;      // [ldk] last column sums b_c
+    float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
     float* cpart;     // [H][ldk] column partial sums
-    double* rmom;     // [rowBlocks * kSaWaves][kSaMomW] per-wave moments
-    double* cby;      // [colBlocks][3] sum_c b_c y_c per column block
+    double* rmom;     // [rowBlocks][kSaMomW] per-workgroup moments of the final pass
+    double* cby;      // [ldk / 64][3] sum_c b_c y_c per finalize block
     double* state;    // kStateW
     double* trace;    // nullable
 };
@@ -172,17 +173,13 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         for (int i = 0; i < 4; ++i) kg[i] = LOG ? (k0 + vv[i]) : k0;  // log: v holds g_c
 #pragma unroll
         for (int i = 0; i < 4; ++i) w[i][0] = LOG ? 1.f : vv[i];
-        if (MODE == 2) {
-            float bb[4];
-            load4(b.bcol + c, bb);
+        if (MODE == 2) {  // precomputed by the last column finalize
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int cc = min(c + i, M - 1);
-                const float base = LOG ? bb[i] : vv[i] * bb[i];
-                const bool in = (c + i) < M;
-                w[i][1 % Q] = in ? base * b.Y[cc] : 0.f;
-                w[i][2 % Q] = in ? base * b.Y[(size_t)M + cc] : 0.f;
-                w[i][3 % Q] = in ? base * b.Y[2 * (size_t)M + cc] : 0.f;
+            for (int q = 1; q < Q; ++q) {
+                float ww[4];
+                load4(b.wfin + (size_t)q * ldk + c, ww);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) w[i][q] = ww[i];
             }
         }
 #pragma unroll
@@ -261,12 +258,20 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             for (int jj = 0; jj < 3; ++jj) m[4 + 3 * i + jj] += s * x[i] * D[jj];
         }
     }
+    __shared__ double wm[kSaWaves][kSaMomW];
     if (lane < kSaMomW) {
         double v = m[0];
 #pragma unroll
         for (int q = 1; q < kSaMomW; ++q)
             if (lane == q) v = m[q];
-        b.rmom[(size_t)gw * kSaMomW + lane] = v;
+        wm[wave][lane] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kSaMomW) {
+        double v = wm[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kSaWaves; ++w) v += wm[w][threadIdx.x];
+        b.rmom[(size_t)blockIdx.x * kSaMomW + threadIdx.x] = v;
     }
 }
 
@@ -324,50 +329,76 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
     b.cpart[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
 }
 
-// Column pass, stage 2: one thread per column sums the row chunks (FP64,
-// fixed order): b_c = (column scale) * sum + o_col[c]; scale /= b_c;
-// o_col /= b_c; b_c kept; sum_c b_c y_c of the block for Yc.
+// Column pass, stage 2: a workgroup finalizes 64 columns; its 4 waves sum
+// interleaved row chunks in FP64 and combine in LDS (fixed order). Then
+// b_c = (column scale) * sum + o_col[c]; scale /= b_c; o_col /= b_c; b_c kept;
+// sum_c b_c y_c of the block for Yc. After the last sweep (last != 0) also the
+// column weights of the final row pass.
+constexpr int kSaFinCols = kWave;
+
 template <bool LOG, bool FIRST>
 __global__ void __launch_bounds__(kWave * kSaWaves)
-sa_colfin_kernel(SaBufs b, int M, int ldk, int H) {
-    const int tid = threadIdx.x;
-    const int cc = blockIdx.x * kSaColsPB + tid;
-    double tot = 0.0;
-    for (int k = 0; k < H; ++k) tot += b.cpart[(size_t)k * ldk + cc];
+sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int cc = blockIdx.x * kSaFinCols + lane;
+    double part = 0.0;
+    for (int k = wave; k < H; k += kSaWaves) part += b.cpart[(size_t)k * ldk + cc];
+    __shared__ double red[kSaWaves][kSaFinCols];
+    red[wave][lane] = part;
+    __syncthreads();
+    if (wave != 0) return;
+    double tot = red[0][lane];
+#pragma unroll
+    for (int w = 1; w < kSaWaves; ++w) tot += red[w][lane];
     double by[3] = {0.0, 0.0, 0.0};
+    float w0 = 0.f, wy[3] = {0.f, 0.f, 0.f};
     if (cc < M) {
         const float sv = FIRST ? (LOG ? 0.f : 1.f) : b.v[cc];
         const float scale = LOG ? exp2f(sv) : sv;
         const float bc = (float)(scale * tot) + b.ocol[cc];
         b.ocol[cc] = b.ocol[cc] / bc;
-        b.v[cc] = LOG ? (sv - log2f(bc)) : (sv / bc);
+        const float vn = LOG ? (sv - log2f(bc)) : (sv / bc);
+        b.v[cc] = vn;
         b.bcol[cc] = bc;
-        by[0] = (double)bc * b.Y[cc];
-        by[1] = (double)bc * b.Y[(size_t)M + cc];
-        by[2] = (double)bc * b.Y[2 * (size_t)M + cc];
+        const float y0 = b.Y[cc], y1 = b.Y[(size_t)M + cc], y2 = b.Y[2 * (size_t)M + cc];
+        by[0] = (double)bc * y0;
+        by[1] = (double)bc * y1;
+        by[2] = (double)bc * y2;
+        // final pass: M_rc b_c y_c = K u_r (v_c b_c) y_c; log-domain keeps v in the exponent
+        w0 = LOG ? 1.f : vn;
+        const float base = w0 * bc;
+        wy[0] = base * y0;
+        wy[1] = base * y1;
+        wy[2] = base * y2;
     } else {
         b.v[cc] = 0.f;
         b.bcol[cc] = 0.f;
     }
-    __shared__ double lds[kSaWaves * 3];
-    double total;
-    block_sum<3>(by, lds, kSaWaves, total);
-    if (tid < 3) b.cby[(size_t)blockIdx.x * 3 + tid] = total;
+    if (last) {
+        b.wfin[cc] = w0;
+        b.wfin[(size_t)ldk + cc] = wy[0];
+        b.wfin[2 * (size_t)ldk + cc] = wy[1];
+        b.wfin[3 * (size_t)ldk + cc] = wy[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) by[i] = wave_sum(by[i]);
+    if (lane < 3) b.cby[(size_t)blockIdx.x * 3 + lane] = (lane == 0) ? by[0] : (lane == 1 ? by[1] : by[2]);
 }
 
 // One workgroup: totals of the per-wave moments and of sum b y (fixed order),
 // S, Horn, the new (R, t), and the transformed sources of the next iteration.
 __global__ void __launch_bounds__(kWave * kSaWaves)
-sa_solve_kernel(SaBufs b, int M, int ldk, int nwaves, int colBlocks, float T, int iter) {
+sa_solve_kernel(SaBufs b, int M, int ldk, int nmom, int nfin, float T, int iter) {
     const int tid = threadIdx.x;
     double m[kSaMomW];
 #pragma unroll
     for (int q = 0; q < kSaMomW; ++q) m[q] = 0.0;
-    for (int w = tid; w < nwaves; w += blockDim.x)
+    for (int w = tid; w < nmom; w += blockDim.x)
 #pragma unroll
         for (int q = 0; q < kSaMomW; ++q) m[q] += b.rmom[(size_t)w * kSaMomW + q];
     double by[3] = {0.0, 0.0, 0.0};
-    for (int k = tid; k < colBlocks; k += blockDim.x)
+    for (int k = tid; k < nfin; k += blockDim.x)
 #pragma unroll
         for (int i = 0; i < 3; ++i) by[i] += b.cby[(size_t)k * 3 + i];
     __shared__ double lds[kSaWaves * kSaMomW];
@@ -429,7 +460,8 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                       hipStream_t s, double* trace) {
     const bool LOGV = (variant == EMSA_SOFTASSIGN_LOGDOMAIN);
     const SaPlan pl = plan_sa(N, M, ctx.numCU);
-    const int nwaves = pl.rowBlocks * kSaWaves;
+    const int nmom = pl.rowBlocks;            // moment rows of the final pass
+    const int finBlocks = pl.ldk / kSaFinCols;  // column-finalize workgroups
     int rc;
     SaBufs b{};
     b.X = dX;
@@ -440,23 +472,23 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         if ((rc = ctx.K.reserve(sizeof(float) * (size_t)N * pl.ldk)) != EMSA_OK) return rc;
         b.K = ctx.K.as<float>();
     }
-    // u, v, bcol each start on a 256-byte boundary (v and bcol are read as float4)
+    // u, v, bcol, wfin each start on a 256-byte boundary (read as float4)
     const size_t npad = ((size_t)N + 63) / 64 * 64;
-    if ((rc = ctx.v.reserve(sizeof(float) * (npad + 2 * (size_t)pl.ldk))) != EMSA_OK) return rc;
+    if ((rc = ctx.v.reserve(sizeof(float) * (npad + 6 * (size_t)pl.ldk))) != EMSA_OK) return rc;
     b.u = ctx.v.as<float>();
     b.v = b.u + npad;
     b.bcol = b.v + pl.ldk;
+    b.wfin = b.bcol + pl.ldk;
     if ((rc = ctx.orow.reserve(sizeof(float) * ((size_t)N + M))) != EMSA_OK) return rc;
     b.orow = ctx.orow.as<float>();
     b.ocol = b.orow + N;
     if ((rc = ctx.colpart.reserve(sizeof(float) * (size_t)pl.H * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
-                                ((size_t)nwaves * kSaMomW + (size_t)pl.colBlocks * 3))) !=
-        EMSA_OK)
+                                ((size_t)nmom * kSaMomW + (size_t)finBlocks * 3))) != EMSA_OK)
         return rc;
     b.rmom = ctx.samom.as<double>();
-    b.cby = b.rmom + (size_t)nwaves * kSaMomW;
+    b.cby = b.rmom + (size_t)nmom * kSaMomW;
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
@@ -465,7 +497,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                        p.moutlier);
     EMSA_HIP_TRY(hipGetLastError());
     const dim3 rgrid((unsigned)pl.rowBlocks), cgrid((unsigned)(pl.colBlocks * pl.H)),
-        fgrid((unsigned)pl.colBlocks);
+        fgrid((unsigned)finBlocks);
     const dim3 blk(kWave * kSaWaves);
     float T = p.T_0;
     int it = 0;
@@ -475,6 +507,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         const float k0 = (float)((double)p.alpha * k1d - 0.5 * std::log2((double)T));
         for (int i0 = 0; i0 < p.I0; ++i0, ++it) {
             for (int sw = 0; sw < p.I1; ++sw) {
+                const int last = (sw + 1 == p.I1) ? 1 : 0;
                 if (LOGV) {
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_rows_kernel<true, 0>), rgrid, blk, 0, s, b, N, M,
@@ -486,10 +519,10 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                                        pl.H, pl.chunkRows, k1, k0);
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_colfin_kernel<true, true>), fgrid, blk, 0, s, b, M,
-                                           pl.ldk, pl.H);
+                                           pl.ldk, pl.H, last);
                     else
                         hipLaunchKernelGGL((sa_colfin_kernel<true, false>), fgrid, blk, 0, s, b,
-                                           M, pl.ldk, pl.H);
+                                           M, pl.ldk, pl.H, last);
                 } else {
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_rows_kernel<false, 0>), rgrid, blk, 0, s, b, N, M,
@@ -501,10 +534,10 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                                        pl.H, pl.chunkRows, k1, k0);
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_colfin_kernel<false, true>), fgrid, blk, 0, s, b,
-                                           M, pl.ldk, pl.H);
+                                           M, pl.ldk, pl.H, last);
                     else
                         hipLaunchKernelGGL((sa_colfin_kernel<false, false>), fgrid, blk, 0, s, b,
-                                           M, pl.ldk, pl.H);
+                                           M, pl.ldk, pl.H, last);
                 }
                 EMSA_HIP_TRY(hipGetLastError());
             }
@@ -514,8 +547,8 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
                                    k1, k0);
-            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), blk, 0, s, b, M, pl.ldk, nwaves,
-                               pl.colBlocks, T, it);
+            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), blk, 0, s, b, M, pl.ldk, nmom,
+                               finBlocks, T, it);
             EMSA_HIP_TRY(hipGetLastError());
         }
         T = T * p.TFACTOR;  // softassign.cu:820, float
