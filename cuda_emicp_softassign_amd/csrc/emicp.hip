@@ -122,7 +122,11 @@ __device__ inline void transform_row(const double* __restrict__ state, float yx,
 
 // Stage 1. COMP: subtract the rounding residual of p_r in every difference,
 // d = (x - p_hi) - p_lo, where x - p_hi is exact for the pairs that matter
-// (Sterbenz), so d carries no error from rounding p_r to FP32.
+// (Sterbenz), so d carries no error from rounding p_r to FP32. Without it the
+// relative error of e is about 1.44 |d| eps |p| / sigma^2 (eps = 6e-8): below
+// 2e-6 for the pairs that carry the normalisers once sigma^2 >= 2e-3, where
+// the 3 subtractions are skipped.
+constexpr float kCompBelowSigma2 = 2e-3f;
 template <bool COMP>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk) {
@@ -491,8 +495,12 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
-        hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
-                           dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
+        if (sig < kCompBelowSigma2)
+            hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
+                               dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
+        else
+            hipLaunchKernelGGL(emicp_affinity_kernel<false>, dim3((unsigned)w.tiles),
+                               dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
         EMSA_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
                            nregs, w.rowBlocks, outlier, last, (int)it);
