@@ -29,7 +29,10 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 1 (B), 6 (W fma)
-VALU_PER_PAIR = 15            # issued per pair: 6 sub (compensated), 3 (d^2), 1 scale, 5 accumulate
+# issued per pair on the VALU at sigma^2 < 2e-3: 6 sub (compensated), 3 (d^2),
+# 1 scale, 1 (C fma); B and W go to the matrix cores (one v_mfma_f32_4x4x1f32
+# per column and wave, i.e. per 64 pairs)
+VALU_PER_PAIR = 11
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
@@ -366,7 +369,8 @@ def main():
                                    for k in range(B)),
         "roofline": {
             "bound": "mfma",
-            "compute": "FP32 VALU + v_exp_f32 (no MFMA: contraction depth 3)",
+            "compute": "FP32 VALU (difference, |d|^2, C) + v_exp_f32; B and W as a product "
+                       "over the column dimension on the matrix cores (v_mfma_f32_4x4x1f32)",
             "kernel": "emicp_affinity_kernel",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
@@ -379,6 +383,7 @@ def main():
             "exp_rate_per_s": pe.value,
             "fma_rate_per_s": pf.value,
             "valu_per_pair": VALU_PER_PAIR,
+            "matrix_instructions_per_pair": 1.0 / 64,
             "pair_rate_per_s": pair_rate,
             "pair_ceiling_per_s": ceiling,
             "pair_ceiling_serial_per_s": ceiling_serial,

@@ -44,6 +44,15 @@ namespace {
 constexpr float kSentinel = 1.0e15f;  // padding target: exp2 underflows to 0 for any sigma
 typedef const __attribute__((address_space(4))) float* cfloat_p;
 
+// Orders LDS writes of some lanes before reads by other lanes of the same
+// wave (a wave's LDS instructions execute in order; this keeps the compiler
+// from moving accesses across and waits for outstanding LDS operations).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Registration served by workgroup b: the last descriptor with tileBase <= b.
 __device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, int b) {
     int lo = 0, hi = nregs - 1;
@@ -201,6 +210,124 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 #pragma unroll
             for (int w = 1; w < kEmWaves; ++w) acc += red[w][q][lane];
             P[q * kEmRows + lane] = acc;
+        }
+    }
+}
+
+// Stage 1 on the matrix cores: B_r = sum_c e_rc and W_r = sum_c e_rc x_c are
+// a product of the e tile (rows x columns) with [x | y | z | 1] (columns x 4),
+// with the column dimension as the contraction. v_mfma_f32_4x4x1f32 works on
+// 16 independent 4x4 blocks of lanes: lane 4b + j, accumulator i receives
+// a(lane 4b + i) * b(lane 4b + j). With a = e (lane = source row, as in the
+// VALU kernel) and b = component j of the current column, lane 4b + j sums
+// component j (x, y, z, 1) for rows 4b .. 4b + 3. The columns' components are
+// staged per wave in LDS as four rows, so one ds_read_b128 gives a lane its
+// component for 4 columns. Per pair on the VALU: the difference, |d|^2, the
+// scale and C += e^2 (11 ops with the compensation, 8 without) + 1 exp, and
+// one matrix instruction per column and wave. UNR columns per loop iteration,
+// NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
+// 16 and 4 are slower from register pressure).
+constexpr int kMfPiece = 256;  // columns staged per wave at a time
+
+template <bool COMP, int UNR, int NACC>
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int ri = find_reg(regs, nregs, blockIdx.x);
+    const EmRegDesc& d = regs[ri];
+    const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
+    const int local = blockIdx.x - d.tileBase;
+    const int rb = local / chunks;
+    const int g = local - rb * chunks;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int r = rb * kEmRows + lane;
+
+    float yx = 0.f, yy = 0.f, yz = 0.f;
+    if (r < M) {
+        yx = d.Y[r];
+        yy = d.Y[(size_t)M + r];
+        yz = d.Y[2 * (size_t)M + r];
+    }
+    double pxd, pyd, pzd;
+    transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
+    const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
+    const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
+                plz = (float)(pzd - (double)pz);
+
+    const cfloat_p Xx = (cfloat_p)d.Xpad;
+    const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
+    const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
+    const int cb = (g * kEmWaves + wave) * waveCols;
+    const int ce = min(cb + waveCols, ldx);
+
+    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
+    float (*st)[kMfPiece] = stage[wave];
+    const int j = lane & 3;  // this lane's component in the matrix product
+    float C = 0.f;
+    f4 D[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) D[a] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int p0 = cb; p0 < ce; p0 += kMfPiece) {
+        const int pn = min(kMfPiece, ce - p0);  // a multiple of kColGroup
+        // stage [x; y; z; 1] of this piece: 4 consecutive columns per lane
+        for (int c = 4 * lane; c < pn; c += 4 * kWave) {
+            const float4 vx = *reinterpret_cast<const float4*>(d.Xpad + p0 + c);
+            const float4 vy = *reinterpret_cast<const float4*>(d.Xpad + ldx + p0 + c);
+            const float4 vz = *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + p0 + c);
+            *reinterpret_cast<float4*>(&st[0][c]) = vx;
+            *reinterpret_cast<float4*>(&st[1][c]) = vy;
+            *reinterpret_cast<float4*>(&st[2][c]) = vz;
+            *reinterpret_cast<float4*>(&st[3][c]) = make_float4(1.f, 1.f, 1.f, 1.f);
+        }
+        wave_lds_sync();
+        for (int c = 0; c < pn; c += UNR) {  // pn is a multiple of kColGroup (and of UNR)
+            f4 bq[UNR / 4];
+#pragma unroll
+            for (int v = 0; v < UNR / 4; ++v)
+                bq[v] = *reinterpret_cast<const f4*>(&st[j][c + 4 * v]);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int cc = p0 + c + u;
+                const float xx = Xx[cc], xy = Xy[cc], xz = Xz[cc];
+                float dx = xx - px, dy = xy - py, dz = xz - pz;
+                if (COMP) {
+                    dx -= plx;
+                    dy -= ply;
+                    dz -= plz;
+                }
+                float s = dx * dx;
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                const float e = __builtin_amdgcn_exp2f(s * negk);
+                C = fmaf(e, e, C);
+                D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[u >> 2][u & 3],
+                                                                 D[u % NACC], 0, 0, 0);
+            }
+        }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int a = 1; a < NACC; ++a) D[0] += D[a];
+
+    // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
+    // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
+    // (slots 2, 3, 4), j = 3 -> B (slot 1).
+    __shared__ float red[kEmWaves][5][kEmRows];
+    red[wave][0][lane] = C;
+    const int q = (j == 3) ? 1 : 2 + j;
+    const int rb4 = lane & ~3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][q][rb4 + i] = D[0][i];
+    __syncthreads();
+    if (wave == 0) {
+        float* P = d.partial + (size_t)local * 5 * kEmRows;
+#pragma unroll
+        for (int qq = 0; qq < 5; ++qq) {
+            float acc = red[0][qq][lane];
+#pragma unroll
+            for (int w = 1; w < kEmWaves; ++w) acc += red[w][qq][lane];
+            P[qq * kEmRows + lane] = acc;
         }
     }
 }
@@ -374,6 +501,29 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
 
 }  // namespace
 
+// The affinity launch of one sigma^2 step: compensated difference below
+// kCompBelowSigma2; B and W on the matrix cores unless the context selects
+// the all-VALU kernel.
+void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
+                     bool comp, float negk, hipStream_t s) {
+    const dim3 blk(kEmRows * kEmWaves);
+    if (ctx.emAffinityMfma) {
+        if (comp)
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk, 0, s,
+                               desc, nregs, negk);
+        else
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk, 0, s,
+                               desc, nregs, negk);
+    } else {
+        if (comp)
+            hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3(tiles), blk, 0, s, desc, nregs,
+                               negk);
+        else
+            hipLaunchKernelGGL(emicp_affinity_kernel<false>, dim3(tiles), blk, 0, s, desc, nregs,
+                               negk);
+    }
+}
+
 EmPlan plan_emicp(int N, int M, int numCU, int blocks_per_cu) {
     EmPlan pl;
     pl.rowBlocks = (M + kEmRows - 1) / kEmRows;
@@ -495,12 +645,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
-        if (sig < kCompBelowSigma2)
-            hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
-                               dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
-        else
-            hipLaunchKernelGGL(emicp_affinity_kernel<false>, dim3((unsigned)w.tiles),
-                               dim3(kEmRows * kEmWaves), 0, s, desc, nregs, negk);
+        launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig < kCompBelowSigma2, negk, s);
         EMSA_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
                            nregs, w.rowBlocks, outlier, last, (int)it);
@@ -518,11 +663,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     return EMSA_OK;
 }
 
-int em_launch_affinity(EmWork& w, float sigma_p2, hipStream_t s) {
+int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s) {
     const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
-    hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3((unsigned)w.tiles),
-                       dim3(kEmRows * kEmWaves), 0, s, w.desc.as<EmRegDesc>(),
-                       (int)w.regs.size(), negk);
+    launch_affinity(ctx, w.desc.as<EmRegDesc>(), (int)w.regs.size(), (unsigned)w.tiles,
+                    sigma_p2 < kCompBelowSigma2, negk, s);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
 }

@@ -132,6 +132,7 @@ struct DeviceCtx {
     hipEvent_t emDone = nullptr;
     bool emPending = false;
     int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
+    int emAffinityMfma = 1;    // 1: B and W of the affinity on the matrix cores; 0: on the VALU
     int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)
 };
 
@@ -167,7 +168,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
 
 // Single launches of the setup and affinity kernels (measurement probes).
 int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s);
-int em_launch_affinity(EmWork& w, float sigma_p2, hipStream_t s);
+int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s);
 int default_tile_cols(const DeviceCtx& ctx);
 
 // Runs the EM-ICP loop on device buffers for one registration (eager).

@@ -117,12 +117,13 @@ extern "C" int emsa_probe_emicp_affinity(const emsa_pair* pairs, int count, int 
     EMSA_HIP_TRY(hipMemcpyAsync(ctx->rt.ptr, rt0.data(), sizeof(float) * rt0.size(),
                                 hipMemcpyHostToDevice, s));
     if ((rc = em_launch_setup(ctx->em, ctx->rt.as<float>(), s)) != EMSA_OK) return rc;
-    if ((rc = em_launch_affinity(ctx->em, sigma_p2, s)) != EMSA_OK) return rc;  // warm-up
+    if ((rc = em_launch_affinity(*ctx, ctx->em, sigma_p2, s)) != EMSA_OK) return rc;  // warm-up
     hipEvent_t e0, e1;
     EMSA_HIP_TRY(hipEventCreate(&e0));
     EMSA_HIP_TRY(hipEventCreate(&e1));
     EMSA_HIP_TRY(hipEventRecord(e0, s));
-    for (int i = 0; i < reps && rc == EMSA_OK; ++i) rc = em_launch_affinity(ctx->em, sigma_p2, s);
+    for (int i = 0; i < reps && rc == EMSA_OK; ++i)
+        rc = em_launch_affinity(*ctx, ctx->em, sigma_p2, s);
     EMSA_HIP_TRY(hipEventRecord(e1, s));
     EMSA_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -133,4 +134,15 @@ extern "C" int emsa_probe_emicp_affinity(const emsa_pair* pairs, int count, int 
     *us_per_launch = 1e3 * (double)ms / reps;
     *pairs_per_launch = np;
     return EMSA_OK;
+}
+
+extern "C" int emsa_probe_set_affinity_path(int use_matrix_cores) {
+    clear_error();
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->emAffinityMfma = use_matrix_cores ? 1 : 0;
+    }
+    return emsa_graph_cache_clear();  // recorded sequences hold the previous kernel
 }

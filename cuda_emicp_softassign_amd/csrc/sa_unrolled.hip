@@ -418,11 +418,11 @@ sau_colpass_kernel(Sau d, const float* __restrict__ z, size_t zb, float k1, floa
     for (int w = 0; w < NW; ++w)
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[w][i] = 0.f;
-    for (int r = rs; r < re; ++r) {
-        const float x0 = X[r], x1 = X[(size_t)N + r], x2 = X[2 * (size_t)N + r];
-        float zw[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) zw[w] = zz[(size_t)w * N + r];
+    // the row index is wave-uniform: rows and their weights come in through
+    // scalar loads, four rows per batch so that the loads are issued together
+    const cfloat_p Xs = (cfloat_p)X;
+    const cfloat_p Zs = (cfloat_p)zz;
+    auto pair4 = [&](float x0, float x1, float x2, const float (&zw)[NW]) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float dx = x0 - p[0][i], dy = x1 - p[1][i], dz = x2 - p[2][i];
@@ -433,6 +433,26 @@ sau_colpass_kernel(Sau d, const float* __restrict__ z, size_t zb, float k1, floa
 #pragma unroll
             for (int w = 0; w < NW; ++w) acc[w][i] = fmaf(K, zw[w], acc[w][i]);
         }
+    };
+    int r = rs;
+    for (; r + 4 <= re; r += 4) {
+        float x0[4], x1[4], x2[4], zw[4][NW];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x0[j] = Xs[r + j];
+            x1[j] = Xs[(size_t)N + r + j];
+            x2[j] = Xs[2 * (size_t)N + r + j];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) zw[j][w] = Zs[(size_t)w * N + r + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pair4(x0[j], x1[j], x2[j], zw[j]);
+    }
+    for (; r < re; ++r) {
+        float zw[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) zw[w] = Zs[(size_t)w * N + r];
+        pair4(Xs[r], Xs[(size_t)N + r], Xs[2 * (size_t)N + r], zw);
     }
     __shared__ float red[kWG][NW][kColsPB];
 #pragma unroll

@@ -27,6 +27,13 @@ int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_st
                               float sigma_p2, int reps, double *us_per_launch,
                               double *pairs_per_launch);
 
+/* Selects, on the current device, where the EM-ICP affinity kernel forms its
+ * row sums B and weighted sums W: 1 (default) on the matrix cores
+ * (v_mfma_f32_4x4x1f32 over the column dimension), 0 on the VALU. Both give
+ * the same sums up to FP32 rounding; for A/B measurement. Drops the recorded
+ * launch sequences of the device. */
+int emsa_probe_set_affinity_path(int use_matrix_cores);
+
 #ifdef __cplusplus
 }
 #endif

@@ -131,3 +131,30 @@ def test_autotune(E):
     Rt, res = E.emicp_batched(pairs[2:3], np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]]))
     assert geodesic(Rt[0, 0, :9].reshape(3, 3).astype(float), R.astype(float)) < 1e-6
     E.graph_cache_clear()
+
+
+def test_affinity_paths_agree(E):
+    """B and W of the affinity on the matrix cores (default) and on the VALU
+    give the same registrations up to FP32 rounding. Only well-conditioned
+    pairs: the 33 x 17 and 60 x 1500 registrations annealed to 2e-4 amplify an
+    FP32 summation-order difference into a different local optimum."""
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    pairs = [_pairs()[k] for k in (1, 2, 4)]
+    p = E.registrationParameters(sigma_inf=2e-4)
+    Rt0 = np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]], np.float32)
+    try:
+        _lib.check("path", L.emsa_probe_set_affinity_path(0))
+        Rv, rv = E.emicp_batched(pairs, Rt0, p)
+        R1v, t1v = E.emicp(pairs[2][0], pairs[2][1], np.eye(3), np.zeros(3), p)
+    finally:
+        _lib.check("path", L.emsa_probe_set_affinity_path(1))
+    Rm, rm = E.emicp_batched(pairs, Rt0, p)
+    R1m, t1m = E.emicp(pairs[2][0], pairs[2][1], np.eye(3), np.zeros(3), p)
+    for k in range(len(pairs)):
+        err = geodesic(Rv[k, 0, :9].reshape(3, 3).astype(float),
+                       Rm[k, 0, :9].reshape(3, 3).astype(float))
+        assert err < 1e-5, (k, err)
+        np.testing.assert_allclose(Rv[k, 0, 9:], Rm[k, 0, 9:], atol=1e-5)
+    np.testing.assert_allclose(rv, rm, rtol=1e-4)
+    assert geodesic(R1v.astype(float), R1m.astype(float)) < 1e-5
