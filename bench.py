@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
 # (8 x 10k, 8e8 pairs): rocprofv3 --pmc FETCH_SIZE (x2, from a 256 MiB copy
 # calibration in the same process) + WRITE_SIZE, run on `bench.py --counters
-# affinity`; 19.7 MiB fetched + 7.7 MiB written
-AFFINITY_TRAFFIC_BYTES = 28.7e6
+# affinity`; 12.4 MiB fetched + 7.7 MiB written
+AFFINITY_TRAFFIC_BYTES = 21.1e6
 
 
 def counters_mode(which, E, _lib, torch):
