@@ -226,7 +226,9 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 // scale and C += e^2 (11 ops with the compensation, 8 without) + 1 exp, and
 // one matrix instruction per column and wave. UNR columns per loop iteration,
 // NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
-// 16 and 4 are slower from register pressure).
+// 16 and 4 are slower from register pressure). One matrix instruction per
+// column and wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
+// the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 
 template <bool COMP, int UNR, int NACC>
@@ -509,11 +511,11 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
     const dim3 blk(kEmRows * kEmWaves);
     if (ctx.emAffinityMfma) {
         if (comp)
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk, 0, s,
-                               desc, nregs, negk);
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk,
+                               0, s, desc, nregs, negk);
         else
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk, 0, s,
-                               desc, nregs, negk);
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk,
+                               0, s, desc, nregs, negk);
     } else {
         if (comp)
             hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3(tiles), blk, 0, s, desc, nregs,
