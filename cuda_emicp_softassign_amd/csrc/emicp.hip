@@ -334,18 +334,11 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     }
 }
 
-// Stage 2: one wave per row block (64 source rows).
-__global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
-                  double outlier, int last, int iter) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const long long gr = (long long)blockIdx.x * kEmWaves + wave;
-    if (gr >= totalRowBlocks) return;  // wave-uniform
-    const int lane = threadIdx.x & (kWave - 1);
-    const int ri = find_reg_rows(regs, nregs, (int)gr);
-    const EmRegDesc& d = regs[ri];
+// Stage 2, one wave and one row block (64 source rows) of registration d:
+// the wave-summed 18 FP64 moments of the block, valid in every lane.
+__device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, double outlier,
+                                         int last, int iter, double (&m)[kMomW]) {
     const int N = d.N, M = d.M, chunks = d.chunks;
-    const int rb = (int)gr - d.rowBase;
     const int r = rb * kEmRows + lane;
     const bool valid = r < M;
     const double* __restrict__ state = d.state;
@@ -415,7 +408,6 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     }
     double pxd, pyd, pzd;
     transform_row(state, yx, yy, yz, pxd, pyd, pzd);
-    double m[kMomW];
     m[0] = lam;
     m[1] = Vx;
     m[2] = Vy;
@@ -438,6 +430,21 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     m[17] = fl;
 #pragma unroll
     for (int q = 0; q < kMomW; ++q) m[q] = wave_sum(m[q]);
+}
+
+// Stage 2: one wave per row block; moments to HBM (moment-major).
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
+                  double outlier, int last, int iter) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long gr = (long long)blockIdx.x * kEmWaves + wave;
+    if (gr >= totalRowBlocks) return;  // wave-uniform
+    const int lane = threadIdx.x & (kWave - 1);
+    const int ri = find_reg_rows(regs, nregs, (int)gr);
+    const EmRegDesc& d = regs[ri];
+    const int rb = (int)gr - d.rowBase;
+    double m[kMomW];
+    row_block_moments(d, rb, lane, outlier, last, iter, m);
     if (lane < kMomW) {
         double v = m[0];
 #pragma unroll
@@ -447,24 +454,10 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     }
 }
 
-// Stage 3: one workgroup per registration.
-__global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
-    const EmRegDesc& d = regs[blockIdx.x];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int rowBlocks = d.rowBlocks;
-    __shared__ double mom[kMomW];
-    // moment q: lanes stride over the row blocks, then a fixed-order wave sum
-    for (int q = wave; q < kMomW; q += kEmWaves) {
-        const double* src = d.moments + (size_t)q * rowBlocks;
-        double s = 0.0;
-        for (int b = lane; b < rowBlocks; b += kWave) s += src[b];
-        s = wave_sum(s);
-        if (lane == 0) mom[q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+// Stage 3 for registration d from its 18 summed moments (one thread):
+// Xc, Yc, S, Horn, the state update and the trace record.
+__device__ inline void solve_registration(const EmRegDesc& d, const double* mom, float sigma_p2,
+                                          int iter) {
     double* __restrict__ state = d.state;
     const double L = mom[0];
     double Xc[3], Yc[3], hS[9], Rn[9], tn[3];
@@ -500,6 +493,28 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
         o[31] = ok ? 0.0 : 1.0;
     }
 }
+
+// Stage 3: one workgroup per registration.
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
+    const EmRegDesc& d = regs[blockIdx.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int rowBlocks = d.rowBlocks;
+    __shared__ double mom[kMomW];
+    // moment q: lanes stride over the row blocks, then a fixed-order wave sum
+    for (int q = wave; q < kMomW; q += kEmWaves) {
+        const double* src = d.moments + (size_t)q * rowBlocks;
+        double s = 0.0;
+        for (int b = lane; b < rowBlocks; b += kWave) s += src[b];
+        s = wave_sum(s);
+        if (lane == 0) mom[q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    solve_registration(d, mom, sigma_p2, iter);
+}
+
 
 }  // namespace
 

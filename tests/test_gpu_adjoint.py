@@ -149,3 +149,52 @@ def test_ckpt_size_and_errors(UR):
         UR.emicp_unrolled(X, X, steps=0)
     with pytest.raises(_lib.EmsaError):
         UR.emicp_unrolled(X, X, sigma2_0=-1.0)
+
+
+def test_adjoint_ragged_batch_through_c_abi(UR):
+    """Pairs of different sizes in one call of the C ABI (the torch wrapper
+    only batches equal sizes): each pair's gradients equal its own run."""
+    import ctypes as C
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    sizes = [(300, 257), (517, 410), (64, 90)]
+    data = [range_scans(n, m, seed=30 + k) for k, (n, m) in enumerate(sizes)]
+    dev = "cuda"
+    Xs = [torch.tensor(np.ascontiguousarray(d[0].T), device=dev) for d in data]
+    Ys = [torch.tensor(np.ascontiguousarray(d[1].T), device=dev) for d in data]
+    B = len(sizes)
+    pairs = (_lib.emsa_pair * B)(*[_lib.emsa_pair(Xs[k].data_ptr(), sizes[k][0], Ys[k].data_ptr(),
+                                                  sizes[k][1]) for k in range(B)])
+    rt0 = torch.tensor(np.tile(IDENT, B), dtype=torch.float32, device=dev)
+    steps = 10
+    n = L.emsa_emicp_unrolled_ckpt_size(pairs, B, steps)
+    ckpt = torch.empty(n, dtype=torch.float64, device=dev)
+    out = torch.empty(12 * B, dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check("fwd", L.emsa_emicp_unrolled_fwd(pairs, B, C.c_void_p(rt0.data_ptr()), 0.01, 0.9,
+                                                0.01, steps, C.c_void_p(out.data_ptr()),
+                                                C.c_void_p(ckpt.data_ptr()), C.c_void_p(s)))
+    g = torch.linspace(-1, 1, 12 * B, device=dev)
+    gX = [torch.empty_like(x) for x in Xs]
+    gY = [torch.empty_like(y) for y in Ys]
+    grads = (_lib.emsa_pair_grad * B)(*[_lib.emsa_pair_grad(gX[k].data_ptr(), gY[k].data_ptr())
+                                        for k in range(B)])
+    gR0 = torch.empty(12 * B, device=dev)
+    gs = torch.empty(B, dtype=torch.float64, device=dev)
+    _lib.check("bwd", L.emsa_emicp_unrolled_bwd(pairs, B, C.c_void_p(rt0.data_ptr()), 0.01, 0.9,
+                                                0.01, steps, C.c_void_p(ckpt.data_ptr()),
+                                                C.c_void_p(g.data_ptr()), grads,
+                                                C.c_void_p(gR0.data_ptr()),
+                                                C.c_void_p(gs.data_ptr()), C.c_void_p(s)))
+    torch.cuda.synchronize()
+    for k in range(B):
+        print("pair", k, sizes[k], flush=True)
+        Xk = torch.tensor(data[k][0], device=dev).requires_grad_(True)
+        Yk = torch.tensor(data[k][1], device=dev).requires_grad_(True)
+        sk = torch.tensor(0.01, dtype=torch.float64, requires_grad=True)
+        Rk = UR.emicp_unrolled(Xk, Yk, None, sk, steps=steps)
+        (Rk * g[12 * k:12 * (k + 1)]).sum().backward()
+        assert torch.allclose(Rk, out[12 * k:12 * (k + 1)], atol=1e-5)
+        assert _rel(gX[k].T.cpu(), Xk.grad.cpu()) < 1e-4
+        assert _rel(gY[k].T.cpu(), Yk.grad.cpu()) < 1e-4
+        assert abs(float(gs[k]) - float(sk.grad)) <= 1e-4 * abs(float(sk.grad)) + 1e-9
