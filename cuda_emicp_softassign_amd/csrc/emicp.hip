@@ -1,4 +1,4 @@
-// EM-ICP on gfx950: one fused kernel launch per sigma^2 step, for one
+// EM-ICP on gfx950: three kernel launches per sigma^2 step, for one
 // registration or for a whole batch of them (uneven sizes, multi-start).
 //
 // Reference semantics (src/emicp.cu:443-668, src/emicp_cpu.cpp:305-527), with
@@ -14,7 +14,8 @@
 //   C_r = sum e^2,  B_r = sum e,  W_r = sum e x_c
 // and then lambda_r = B_r / sqrt(C_r), lambda_r X'_r = W_r / sqrt(C_r).
 // Target points are wave-uniform, so they are read with scalar loads and
-// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair.
+// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair, or 11
+// when B and W are summed on the matrix cores (the default kernel).
 //
 // Work decomposition per sigma^2 step, three launches with no signalling
 // between workgroups (kernel boundaries order them):
@@ -546,7 +547,9 @@ EmPlan plan_emicp(int N, int M, int numCU, int blocks_per_cu) {
     pl.rowBlocks = (M + kEmRows - 1) / kEmRows;
     pl.ldx = (N + kColGroup - 1) / kColGroup * kColGroup;
     const int groups = pl.ldx / kColGroup;  // column groups of 16
-    const int target = std::max(1, numCU) * (blocks_per_cu > 0 ? blocks_per_cu : 8);
+    // 12 workgroups per CU: measured best for a plain 10k registration (more
+    // chunks speed up the affinity but lengthen the row finalize)
+    const int target = std::max(1, numCU) * (blocks_per_cu > 0 ? blocks_per_cu : 12);
     int chunks = (target + pl.rowBlocks - 1) / pl.rowBlocks;
     chunks = std::max(1, std::min(chunks, (groups + kEmWaves - 1) / kEmWaves));
     const int gpw = (groups + chunks * kEmWaves - 1) / (chunks * kEmWaves);  // groups per wave
