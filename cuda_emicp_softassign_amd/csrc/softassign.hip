@@ -40,16 +40,15 @@ namespace emsa {
 
 namespace {
 
-constexpr int kSaRW = 2;                     // rows per wave in row passes
 constexpr int kSaWaves = 4;                  // waves per workgroup
-constexpr int kSaRowsPB = kSaRW * kSaWaves;  // rows per row-pass workgroup
+constexpr int kSaRowsWG = 8;                 // rows per row-pass workgroup
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
 constexpr int kSaMomW = 22;                  // per-wave moments of the final pass
 constexpr float kSaSentinel = 1.0e15f;
 
 struct SaPlan {
     int ldk = 0;        // padded column count (multiple of 256)
-    int rowBlocks = 0;  // ceil(N / kSaRowsPB) row-pass workgroups
+    int rowBlocks = 0;  // ceil(N / kSaRowsWG) row-pass workgroups
     int colBlocks = 0;  // ldk / 256
     int H = 0;          // row chunks per column block
     int chunkRows = 0;  // rows per chunk (multiple of kSaWaves)
@@ -58,7 +57,7 @@ struct SaPlan {
 SaPlan plan_sa(int N, int M, int numCU) {
     SaPlan p;
     p.ldk = (M + kSaColsPB - 1) / kSaColsPB * kSaColsPB;
-    p.rowBlocks = (N + kSaRowsPB - 1) / kSaRowsPB;
+    p.rowBlocks = (N + kSaRowsWG - 1) / kSaRowsWG;
     p.colBlocks = p.ldk / kSaColsPB;
     const int target = std::max(1, numCU) * 4;
     int H = std::max(1, (target + p.colBlocks - 1) / p.colBlocks);
@@ -129,35 +128,40 @@ __device__ inline void load4(const float* p, float (&v)[4]) {
 }
 
 // ---------------------------------------------------------------------------
-// Row passes: each wave owns kSaRW whole rows, lanes along columns (4
-// consecutive columns per lane). MODE 0: first row pass of an inner iteration
-// (materialised: also writes K); MODE 1: Sinkhorn row pass; MODE 2: final
-// pass (s_r, D'_r and the wave's FP64 moments).
+// Row passes: a workgroup owns kSaRowsWG whole rows; its 4 waves take
+// interleaved slabs of 256 columns (lanes: 4 consecutive columns each), so
+// every load of the transformed sources P and the column scale serves all
+// kSaRowsWG rows. Row sums: a DPP reduction per wave into lane 63, then the 4
+// waves combine in LDS (fixed order). MODE 0: first row pass of an inner
+// iteration (materialised: also writes K); MODE 1: Sinkhorn row pass; MODE 2:
+// final pass (s_r, D'_r and the workgroup's FP64 moments).
 // ---------------------------------------------------------------------------
+
 template <bool LOG, int MODE>
 __global__ void __launch_bounds__(kWave * kSaWaves)
 sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     constexpr int Q = (MODE == 2) ? 4 : 1;
+    constexpr int R = kSaRowsWG;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int gw = blockIdx.x * kSaWaves + wave;
-    const int r0 = gw * kSaRW;
+    const int r0 = blockIdx.x * R;
+    const int nr = min(R, N - r0);  // rows of this workgroup (uniform)
 
-    float xr[kSaRW][3];
+    float xr[R][3];
 #pragma unroll
-    for (int j = 0; j < kSaRW; ++j) {
-        const int r = min(r0 + j, N - 1);
+    for (int j = 0; j < R; ++j) {
+        const int r = r0 + min(j, nr - 1);
         xr[j][0] = b.X[r];
         xr[j][1] = b.X[(size_t)N + r];
         xr[j][2] = b.X[2 * (size_t)N + r];
     }
-    float acc[kSaRW][Q];
+    float acc[R][Q];
 #pragma unroll
-    for (int j = 0; j < kSaRW; ++j)
+    for (int j = 0; j < R; ++j)
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[j][q] = 0.f;
 
-    for (int c = 4 * lane; c < ldk; c += 4 * kWave) {
+    for (int c = 4 * lane + 4 * kWave * wave; c < ldk; c += 4 * kWave * kSaWaves) {
         float w[4][Q];  // per-column weights
         float p[3][4];
         float kg[4];
@@ -183,9 +187,9 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             }
         }
 #pragma unroll
-        for (int j = 0; j < kSaRW; ++j) {
+        for (int j = 0; j < R; ++j) {
+            if (j >= nr) continue;  // uniform
             const int r = r0 + j;
-            if (r >= N) continue;  // wave-uniform
             float val[4];
             if (!LOG && MODE != 0) {
                 load4(b.K + (size_t)r * ldk + c, val);
@@ -210,69 +214,62 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
                 for (int q = 0; q < Q; ++q) acc[j][q] = fmaf(val[i], w[i][q], acc[j][q]);
         }
     }
+    // row sums: lane 63 of each wave, then the 4 waves in a fixed order
+    __shared__ float red[kSaWaves][R * Q];
 #pragma unroll
-    for (int j = 0; j < kSaRW; ++j)
+    for (int j = 0; j < R; ++j)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) acc[j][q] = wave_sum(acc[j][q]);
+        for (int q = 0; q < Q; ++q) {
+            const float t = wave_sum_last(acc[j][q]);
+            if (lane == kWave - 1) red[wave][j * Q + q] = t;
+        }
+    __syncthreads();
+    __shared__ float tot[R * Q];
+    if (threadIdx.x < R * Q) {
+        float v = red[0][threadIdx.x];
+#pragma unroll
+        for (int w2 = 1; w2 < kSaWaves; ++w2) v += red[w2][threadIdx.x];
+        tot[threadIdx.x] = v;
+    }
+    __syncthreads();
 
     if (MODE != 2) {
         // a_r = (row scale) * sum + o_row[r]; scale /= a_r; o_row /= a_r
-        if (lane < kSaRW) {
-            float tot = acc[0][0];
-#pragma unroll
-            for (int j = 1; j < kSaRW; ++j)
-                if (lane == j) tot = acc[j][0];
-            const int r = r0 + lane;
-            if (r < N) {
-                const float su = (MODE == 0) ? (LOG ? 0.f : 1.f) : b.u[r];
-                const float scale = LOG ? exp2f(su) : su;
-                const float a = scale * tot + b.orow[r];
-                b.orow[r] = b.orow[r] / a;
-                b.u[r] = LOG ? (su - log2f(a)) : (su / a);
-            }
+        const int j = threadIdx.x;
+        if (j < nr) {
+            const int r = r0 + j;
+            const float su = (MODE == 0) ? (LOG ? 0.f : 1.f) : b.u[r];
+            const float scale = LOG ? exp2f(su) : su;
+            const float a = scale * tot[j] + b.orow[r];
+            b.orow[r] = b.orow[r] / a;
+            b.u[r] = LOG ? (su - log2f(a)) : (su / a);
         }
         return;
     }
 
-    // MODE 2: FP64 moments of this wave's rows
+    // MODE 2: FP64 moments of this workgroup's rows, in row order (thread 0)
     //  [0] s  [1..3] s x  [4 + 3i + j] s x_i D'_j  [13..15] s^2 x  [16..18] D'
+    if (threadIdx.x != 0) return;
     double m[kSaMomW];
 #pragma unroll
     for (int q = 0; q < kSaMomW; ++q) m[q] = 0.0;
-#pragma unroll
-    for (int j = 0; j < kSaRW; ++j) {
+    for (int j = 0; j < nr; ++j) {
         const int r = r0 + j;
-        if (r >= N) continue;
         const float su = b.u[r];
         const double scale = LOG ? (double)exp2f(su) : (double)su;
-        const double s = scale * acc[j][0];
-        const double D[3] = {scale * acc[j][1], scale * acc[j][2], scale * acc[j][3]};
-        const double x[3] = {xr[j][0], xr[j][1], xr[j][2]};
+        const double s = scale * tot[j * Q];
+        const double D[3] = {scale * tot[j * Q + 1], scale * tot[j * Q + 2],
+                             scale * tot[j * Q + 3]};
+        const double x[3] = {b.X[r], b.X[(size_t)N + r], b.X[2 * (size_t)N + r]};
         m[0] += s;
-#pragma unroll
         for (int i = 0; i < 3; ++i) {
             m[1 + i] += s * x[i];
             m[13 + i] += s * s * x[i];
             m[16 + i] += D[i];
-#pragma unroll
             for (int jj = 0; jj < 3; ++jj) m[4 + 3 * i + jj] += s * x[i] * D[jj];
         }
     }
-    __shared__ double wm[kSaWaves][kSaMomW];
-    if (lane < kSaMomW) {
-        double v = m[0];
-#pragma unroll
-        for (int q = 1; q < kSaMomW; ++q)
-            if (lane == q) v = m[q];
-        wm[wave][lane] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kSaMomW) {
-        double v = wm[0][threadIdx.x];
-#pragma unroll
-        for (int w = 1; w < kSaWaves; ++w) v += wm[w][threadIdx.x];
-        b.rmom[(size_t)blockIdx.x * kSaMomW + threadIdx.x] = v;
-    }
+    for (int q = 0; q < kSaMomW; ++q) b.rmom[(size_t)blockIdx.x * kSaMomW + q] = m[q];
 }
 
 // ---------------------------------------------------------------------------
