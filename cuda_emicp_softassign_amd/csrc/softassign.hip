@@ -41,7 +41,7 @@ namespace emsa {
 namespace {
 
 constexpr int kSaWaves = 4;                  // waves per workgroup
-constexpr int kSaRowsWG = 8;                 // rows per row-pass workgroup
+constexpr int kSaRowsWG = 8;                 // rows per row-pass workgroup (16: too few workgroups)
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
 constexpr int kSaMomW = 22;                  // per-wave moments of the final pass
 constexpr float kSaSentinel = 1.0e15f;
