@@ -76,6 +76,7 @@ SIGNATURES = {
     "emsa_probe_emicp_affinity": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_int, C.c_float,
                                             C.c_int, _dp, _dp]),
     "emsa_probe_set_affinity_path": (C.c_int, [C.c_int]),
+    "emsa_probe_set_affinity_cull": (C.c_int, [C.c_int]),
     "emsa_emicp_unrolled_ckpt_size": (C.c_longlong, [C.POINTER(emsa_pair), C.c_int, C.c_int]),
     "emsa_emicp_unrolled_fwd": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_void_p, C.c_float,
                                           C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p,

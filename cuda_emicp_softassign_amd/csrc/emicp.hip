@@ -35,6 +35,7 @@
 #include "emsa_internal.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -65,19 +66,197 @@ __device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, in
     return lo;
 }
 
-// One workgroup per registration: pad the target (once per target cloud),
-// sum it in FP64 (for floored rows) and load the initial transform.
+// ---------------------------------------------------------------------------
+// Working order of the clouds. Each cloud (the target and the source of every
+// pair) is put in the order of a 3D Hilbert curve over its bounding box
+// (kOrdBits bits per axis), so that consecutive points are close in space: a
+// wave's 64 source rows and each group of kCullCols target columns then have
+// small bounding boxes. The affinity kernel skips a column group for a wave
+// when the boxes are so far apart that exp2(-|d|^2 k) underflows to exactly
+// 0 for every pair of the group (e = 0 adds nothing to C, B or W), so every
+// sum stays bit for bit what the full loop over the same order gives. Per-row
+// results are written back in the caller's row order.
+// ---------------------------------------------------------------------------
+constexpr int kOrdThreads = 1024;
+constexpr int kOrdWaves = kOrdThreads / kWave;
+constexpr int kOrdBits = 10;
+// |exp2 argument| beyond which e is 0 in FP32 (2^-150 already rounds to 0);
+// the margin covers the rounding of the box distance.
+constexpr float kCullArg = 160.f;
+
+// Index of cell (x, y, z) (kOrdBits bits each) along the 3D Hilbert curve:
+// J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707 (2004):
+// the axes are turned into the transposed index, whose bits are then
+// interleaved from the most significant level down.
+__host__ __device__ inline unsigned hilbert_key(unsigned x, unsigned y, unsigned z) {
+    unsigned v[3] = {x, y, z};
+    for (unsigned q = 1u << (kOrdBits - 1); q > 1; q >>= 1) {
+        const unsigned p = q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (v[i] & q) {
+                v[0] ^= p;
+            } else {
+                const unsigned t = (v[0] ^ v[i]) & p;
+                v[0] ^= t;
+                v[i] ^= t;
+            }
+        }
+    }
+    v[1] ^= v[0];
+    v[2] ^= v[1];
+    unsigned t = 0;
+    for (unsigned q = 1u << (kOrdBits - 1); q > 1; q >>= 1)
+        if (v[2] & q) t ^= q - 1;
+    unsigned key = 0;
+    for (int b = kOrdBits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | (((v[i] ^ t) >> b) & 1u);
+    return key;
+}
+
+// One workgroup per cloud: blockIdx.x = 2 * registration + (0: target,
+// 1: source); only the registration that owns a pair's buffers orders it.
+// Writes the permutation (working position -> caller's index): the Hilbert
+// key of every point, then a stable least-significant-digit radix sort of
+// (key, index), 8 bits per pass. A pass histograms the digits, then scatters
+// tiles of kOrdThreads keys in order; within a tile, equal digits are ranked
+// inside a wave with ballots and across waves with per-digit prefix counts in
+// LDS. Non-finite points get the largest key.
+__global__ void __launch_bounds__(kOrdThreads)
+emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
+    const EmRegDesc& d = regs[blockIdx.x >> 1];
+    if (!d.padOwner || !d.rowperm) return;  // workgroup-uniform
+    const bool src = (blockIdx.x & 1) != 0;
+    const int n = src ? d.M : d.N;
+    const float* __restrict__ P = src ? d.Y : d.X;
+    int* __restrict__ perm = src ? d.rowperm : d.colperm;
+    unsigned* kA = d.sortTmp + (src ? 4 * (size_t)d.N : 0);
+    unsigned* vA = kA + n;
+    unsigned* kB = vA + n;
+    unsigned* vB = kB + n;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+
+    __shared__ float boxs[kOrdWaves][6];
+    __shared__ unsigned base[256], tot[256];
+    __shared__ unsigned wcnt[kOrdWaves * 256];
+
+    // bounding box of the finite points
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = tid; i < n; i += kOrdThreads)
+        for (int a = 0; a < 3; ++a) {
+            const float v = P[(size_t)a * n + i];
+            if (fabsf(v) <= FLT_MAX) {
+                lo[a] = fminf(lo[a], v);
+                hi[a] = fmaxf(hi[a], v);
+            }
+        }
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        hi[a] = wave_max(hi[a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            boxs[wave][a] = lo[a];
+            boxs[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    float scale[3];
+    for (int a = 0; a < 3; ++a) {
+        for (int w = 0; w < kOrdWaves; ++w) {
+            lo[a] = fminf(lo[a], boxs[w][a]);
+            hi[a] = fmaxf(hi[a], boxs[w][3 + a]);
+        }
+        scale[a] = (hi[a] > lo[a]) ? (float)((1 << kOrdBits) - 1) / (hi[a] - lo[a]) : 0.f;
+    }
+    for (int i = tid; i < n; i += kOrdThreads) {
+        unsigned q[3];
+        bool fin = true;
+        for (int a = 0; a < 3; ++a) {
+            const float v = P[(size_t)a * n + i];
+            fin = fin && fabsf(v) <= FLT_MAX;
+            // NaN from an infinite range times a zero scale clamps to 0
+            const float f = fminf(fmaxf((v - lo[a]) * scale[a], 0.f), (float)((1 << kOrdBits) - 1));
+            q[a] = (unsigned)f;
+        }
+        kA[i] = fin ? hilbert_key(q[0], q[1], q[2]) : 0xFFFFFFFFu;
+        vA[i] = (unsigned)i;
+    }
+    __syncthreads();
+
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int sh = 8 * pass;
+        const unsigned* ki = (pass & 1) ? kB : kA;
+        const unsigned* vi = (pass & 1) ? vB : vA;
+        unsigned* ko = (pass & 1) ? kA : kB;
+        unsigned* vo = (pass & 1) ? vA : vB;
+        if (tid < 256) tot[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += kOrdThreads) atomicAdd(&tot[(ki[i] >> sh) & 255u], 1u);
+        __syncthreads();
+        if (tid == 0) {
+            unsigned run = 0;
+            for (int b = 0; b < 256; ++b) {
+                base[b] = run;
+                run += tot[b];
+            }
+        }
+        for (int t0 = 0; t0 < n; t0 += kOrdThreads) {
+            const int i = t0 + tid;
+            const bool valid = i < n;
+            const unsigned key = valid ? ki[i] : 0u;
+            const unsigned val = valid ? vi[i] : 0u;
+            const unsigned dig = (key >> sh) & 255u;
+            // lanes of this wave holding the same digit
+            unsigned long long same = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = ((dig >> b) & 1u) != 0;
+                const unsigned long long bb = __ballot(bit);
+                same &= bit ? bb : ~bb;
+            }
+            const unsigned rank = (unsigned)__popcll(same & below);
+            for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
+            __syncthreads();
+            if (valid && rank == 0) wcnt[wave * 256 + dig] = (unsigned)__popcll(same);
+            __syncthreads();
+            if (tid < 256) {  // per digit: offsets of the waves, then the tile total
+                unsigned run = 0;
+                for (int w = 0; w < kOrdWaves; ++w) {
+                    const unsigned c = wcnt[w * 256 + tid];
+                    wcnt[w * 256 + tid] = run;
+                    run += c;
+                }
+                tot[tid] = run;
+            }
+            __syncthreads();
+            if (valid) {
+                const unsigned pos = base[dig] + wcnt[wave * 256 + dig] + rank;
+                ko[pos] = key;
+                vo[pos] = val;
+            }
+            __syncthreads();
+            if (tid < 256) base[tid] += tot[tid];
+        }
+        __syncthreads();  // this pass's output is the next pass's input
+    }
+    for (int i = tid; i < n; i += kOrdThreads) perm[i] = (int)vA[i];
+}
+
+// One workgroup per registration: pad the target in working order (once per
+// target cloud), with the boxes of its column groups and the source in
+// working order; sum the target in FP64 (for floored rows) and load the
+// initial transform.
 __global__ void __launch_bounds__(256)
 emicp_setup_kernel(const EmRegDesc* __restrict__ regs, const float* __restrict__ Rt0) {
     const EmRegDesc& d = regs[blockIdx.x];
-    const int N = d.N, ldx = d.ldx;
+    const int N = d.N, M = d.M, ldx = d.ldx;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int c = threadIdx.x; c < ldx; c += blockDim.x) {
         float x = kSentinel, y = kSentinel, z = kSentinel;
         if (c < N) {
-            x = d.X[c];
-            y = d.X[(size_t)N + c];
-            z = d.X[2 * (size_t)N + c];
+            const int o = d.colperm ? d.colperm[c] : c;
+            x = d.X[o];
+            y = d.X[(size_t)N + o];
+            z = d.X[2 * (size_t)N + o];
             sx += x;
             sy += y;
             sz += z;
@@ -86,6 +265,40 @@ emicp_setup_kernel(const EmRegDesc* __restrict__ regs, const float* __restrict__
             d.Xpad[c] = x;
             d.Xpad[(size_t)ldx + c] = y;
             d.Xpad[2 * (size_t)ldx + c] = z;
+        }
+    }
+    if (d.padOwner && d.rowperm) {
+        for (int r = threadIdx.x; r < M; r += blockDim.x) {
+            const int o = d.rowperm[r];
+            d.Ys[r] = d.Y[o];
+            d.Ys[(size_t)M + r] = d.Y[(size_t)M + o];
+            d.Ys[2 * (size_t)M + r] = d.Y[2 * (size_t)M + o];
+        }
+    }
+    __syncthreads();  // the group boxes read the padded target written above
+    if (d.padOwner && d.gbox) {
+        for (int g = threadIdx.x; g < ldx / kCullCols; g += blockDim.x) {
+            // empty group (padding only): lo > hi, always skipped; a group with a
+            // non-finite point: the whole space, never skipped
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            bool fin = true;
+            for (int u = 0; u < kCullCols; ++u) {
+                const int c = g * kCullCols + u;
+                if (c >= N) break;
+                for (int a = 0; a < 3; ++a) {
+                    const float v = d.Xpad[(size_t)a * ldx + c];
+                    fin = fin && fabsf(v) <= FLT_MAX;
+                    lo[a] = fminf(lo[a], v);
+                    hi[a] = fmaxf(hi[a], v);
+                }
+            }
+            float* b = d.gbox + (size_t)g * 8;
+            for (int a = 0; a < 3; ++a) {
+                b[a] = fin ? lo[a] : -INFINITY;
+                b[4 + a] = fin ? hi[a] : INFINITY;
+            }
+            b[3] = 0.f;
+            b[7] = 0.f;
         }
     }
     __shared__ double lds[4 * 3];
@@ -153,9 +366,9 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 
     float yx = 0.f, yy = 0.f, yz = 0.f;
     if (r < M) {
-        yx = d.Y[r];
-        yy = d.Y[(size_t)M + r];
-        yz = d.Y[2 * (size_t)M + r];
+        yx = d.Ys[r];
+        yy = d.Ys[(size_t)M + r];
+        yz = d.Ys[2 * (size_t)M + r];
     }
     double pxd, pyd, pzd;
     transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
@@ -230,11 +443,20 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 // 16 and 4 are slower from register pressure). One matrix instruction per
 // column and wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
 // the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
+// cull != 0 (and boxes present): per piece, each of the first pn / 8 lanes
+// tests one 8-column group's box against the box of the wave's valid rows,
+// and the loop visits only the groups with a pair whose exp2 argument can be
+// above -kCullArg (a ballot mask, walked lowest bit first); a piece with no
+// such group is not even staged. visits (nullable, measurement): receives the
+// number of (valid row, valid column) pairs the launch evaluates.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 
 template <bool COMP, int UNR, int NACC>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk) {
+emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, int cull,
+                           unsigned long long* __restrict__ visits) {
+    static_assert(UNR == kCullCols, "one loop iteration per column group");
+    static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
     const int ri = find_reg(regs, nregs, blockIdx.x);
     const EmRegDesc& d = regs[ri];
@@ -248,9 +470,9 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 
     float yx = 0.f, yy = 0.f, yz = 0.f;
     if (r < M) {
-        yx = d.Y[r];
-        yy = d.Y[(size_t)M + r];
-        yz = d.Y[2 * (size_t)M + r];
+        yx = d.Ys[r];
+        yy = d.Ys[(size_t)M + r];
+        yz = d.Ys[2 * (size_t)M + r];
     }
     double pxd, pyd, pzd;
     transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
@@ -264,6 +486,25 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     const int cb = (g * kEmWaves + wave) * waveCols;
     const int ce = min(cb + waveCols, ldx);
 
+    // box of the wave's valid rows; a non-finite row (NaN affinities in every
+    // column) turns the skipping off for the wave
+    const bool rv = r < M;
+    const float nk = -negk;
+    bool doCull = cull && d.gbox;
+    float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
+    if (doCull) {
+        const bool bad = rv && !(fabsf(px) <= FLT_MAX && fabsf(py) <= FLT_MAX && fabsf(pz) <= FLT_MAX);
+        doCull = __ballot(bad) == 0;
+        const float pv[3] = {px, py, pz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            rlo[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(wave_min(rv ? pv[a] : INFINITY))));
+            rhi[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(wave_max(rv ? pv[a] : -INFINITY))));
+        }
+    }
+
     __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
     float (*st)[kMfPiece] = stage[wave];
     const int j = lane & 3;  // this lane's component in the matrix product
@@ -271,8 +512,26 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     f4 D[NACC];
 #pragma unroll
     for (int a = 0; a < NACC; ++a) D[a] = f4{0.f, 0.f, 0.f, 0.f};
+    int nvis = 0;  // valid columns visited by this wave
     for (int p0 = cb; p0 < ce; p0 += kMfPiece) {
         const int pn = min(kMfPiece, ce - p0);  // a multiple of kColGroup
+        const int ng = pn / kCullCols;
+        unsigned keep = (ng >= 32) ? 0xFFFFFFFFu : ((1u << ng) - 1u);
+        if (doCull) {
+            bool kg = false;
+            if (lane < ng) {
+                const float* bx = d.gbox + (size_t)(p0 / kCullCols + lane) * 8;
+                const float4 lo = *reinterpret_cast<const float4*>(bx);
+                const float4 hi = *reinterpret_cast<const float4*>(bx + 4);
+                const float gx = fmaxf(fmaxf(rlo[0] - hi.x, lo.x - rhi[0]), 0.f);
+                const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
+                const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
+                const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+                kg = !(g2 * nk > kCullArg);
+            }
+            keep = (unsigned)__ballot(kg);
+            if (keep == 0) continue;
+        }
         // stage [x; y; z; 1] of this piece: 4 consecutive columns per lane
         for (int c = 4 * lane; c < pn; c += 4 * kWave) {
             const float4 vx = *reinterpret_cast<const float4*>(d.Xpad + p0 + c);
@@ -284,7 +543,10 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
             *reinterpret_cast<float4*>(&st[3][c]) = make_float4(1.f, 1.f, 1.f, 1.f);
         }
         wave_lds_sync();
-        for (int c = 0; c < pn; c += UNR) {  // pn is a multiple of kColGroup (and of UNR)
+        while (keep) {  // wave-uniform: the groups of UNR columns still to visit
+            const int c = __builtin_ctz(keep) * UNR;
+            keep &= keep - 1u;
+            if (visits) nvis += max(0, min(UNR, d.N - (p0 + c)));
             f4 bq[UNR / 4];
 #pragma unroll
             for (int v = 0; v < UNR / 4; ++v)
@@ -312,6 +574,8 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     }
 #pragma unroll
     for (int a = 1; a < NACC; ++a) D[0] += D[a];
+    if (visits && lane == 0 && nvis > 0)
+        atomicAdd(visits, (unsigned long long)nvis * (unsigned long long)min(kEmRows, M - rb * kEmRows));
 
     // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
     // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
@@ -388,19 +652,21 @@ __device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, d
         fl = 1.0;
     }
     float yx = 0.f, yy = 0.f, yz = 0.f;
+    int ro = r;  // the caller's index of this row
     if (valid) {
-        yx = d.Y[r];
-        yy = d.Y[(size_t)M + r];
-        yz = d.Y[2 * (size_t)M + r];
+        yx = d.Ys[r];
+        yy = d.Ys[(size_t)M + r];
+        yz = d.Ys[2 * (size_t)M + r];
+        if (d.rowperm) ro = d.rowperm[r];
     } else {
         lam = Vx = Vy = Vz = fl = 0.0;
     }
     if (last && d.rows && valid) {
-        d.rows[r] = Cs;
-        d.rows[(size_t)M + r] = lam;
+        d.rows[ro] = Cs;
+        d.rows[(size_t)M + ro] = lam;
     }
     if (d.ckpt && valid) {
-        double* ck = d.ckpt + ((size_t)iter * M + r) * 5;
+        double* ck = d.ckpt + ((size_t)iter * M + ro) * 5;
         ck[0] = Cs;
         ck[1] = lam;
         ck[2] = Vx;
@@ -523,15 +789,15 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
 // kCompBelowSigma2; B and W on the matrix cores unless the context selects
 // the all-VALU kernel.
 void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
-                     bool comp, float negk, hipStream_t s) {
+                     bool comp, float negk, hipStream_t s, unsigned long long* visits) {
     const dim3 blk(kEmRows * kEmWaves);
     if (ctx.emAffinityMfma) {
         if (comp)
             hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk,
-                               0, s, desc, nregs, negk);
+                               0, s, desc, nregs, negk, ctx.emCull, visits);
         else
             hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk,
-                               0, s, desc, nregs, negk);
+                               0, s, desc, nregs, negk, ctx.emCull, visits);
     } else {
         if (comp)
             hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3(tiles), blk, 0, s, desc, nregs,
@@ -578,9 +844,13 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
         ctx.emPending = false;
     }
     w.regs.clear();
-    size_t xpadF = 0, partF = 0, momD = 0;
+    // working order (Hilbert) and column-group boxes, except for the unrolled
+    // forward, whose per-step checkpoints follow the caller's row order
+    const bool order = (aux == nullptr);
+    size_t xpadF = 0, partF = 0, momD = 0, ordW = 0;
     long long tiles = 0, rowBlocks = 0;
-    std::vector<size_t> xoff(count);
+    std::vector<size_t> xoff(count), ooff(count);
+    auto al64 = [](size_t v) { return (v + 63) / 64 * 64; };
     for (int k = 0; k < count; ++k) {
         const EmPair& pr = pairs[k];
         if (!pr.X || !pr.Y || pr.N <= 0 || pr.M <= 0)
@@ -596,6 +866,12 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
         }
         xoff[k] = xpadF;
         xpadF += 3 * (size_t)pl.ldx;
+        // per pair, in 4-byte words: boxes [ldx], Ys [3M], rowperm [M],
+        // colperm [N], ordering scratch [4N + 4M]
+        ooff[k] = ordW;
+        if (order)
+            ordW += al64(pl.ldx) + al64(3 * (size_t)pr.M) + al64(pr.M) + al64(pr.N) +
+                    al64(4 * ((size_t)pr.N + pr.M));
         for (int st = 0; st < n_starts; ++st) {
             EmRegDesc d{};
             d.X = pr.X;
@@ -639,10 +915,28 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     if ((rc = w.moments.reserve(sizeof(double) * momD)) != EMSA_OK) return rc;
     if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
     if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
+    if (order && (rc = w.order.reserve(4 * ordW)) != EMSA_OK) return rc;
     for (int i = 0; i < nregs; ++i) {
         EmRegDesc& d = w.regs[i];
         const int k = d.slot / n_starts;
         d.Xpad = w.xpad.as<float>() + xoff[k];
+        if (order) {
+            float* o = w.order.as<float>() + ooff[k];
+            d.gbox = o;
+            o += al64(d.ldx);
+            d.Ys = o;
+            o += al64(3 * (size_t)d.M);
+            d.rowperm = reinterpret_cast<int*>(o);
+            o += al64(d.M);
+            d.colperm = reinterpret_cast<int*>(o);
+            o += al64(d.N);
+            d.sortTmp = reinterpret_cast<unsigned*>(o);
+        } else {
+            d.Ys = const_cast<float*>(d.Y);
+            d.gbox = nullptr;
+            d.rowperm = d.colperm = nullptr;
+            d.sortTmp = nullptr;
+        }
         d.partial = w.partial.as<float>() + reinterpret_cast<size_t>(d.partial);
         d.moments = w.moments.as<double>() + reinterpret_cast<size_t>(d.moments);
         d.state = w.state.as<double>() + (size_t)i * kStateW;
@@ -657,15 +951,16 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
                const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s) {
     const int nregs = (int)w.regs.size();
     const EmRegDesc* desc = w.desc.as<EmRegDesc>();
-    hipLaunchKernelGGL(emicp_setup_kernel, dim3(nregs), dim3(256), 0, s, desc, dRt0);
-    EMSA_HIP_TRY(hipGetLastError());
+    int rc = em_launch_setup(w, dRt0, s);
+    if (rc != EMSA_OK) return rc;
     const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
-        launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig < kCompBelowSigma2, negk, s);
+        launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig < kCompBelowSigma2, negk, s,
+                        nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
                            nregs, w.rowBlocks, outlier, last, (int)it);
@@ -683,16 +978,23 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     return EMSA_OK;
 }
 
-int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s) {
+int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s,
+                       unsigned long long* visits) {
     const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
     launch_affinity(ctx, w.desc.as<EmRegDesc>(), (int)w.regs.size(), (unsigned)w.tiles,
-                    sigma_p2 < kCompBelowSigma2, negk, s);
+                    sigma_p2 < kCompBelowSigma2, negk, s, visits);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
 }
 
 int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s) {
-    hipLaunchKernelGGL(emicp_setup_kernel, dim3((unsigned)w.regs.size()), dim3(256), 0, s,
+    const unsigned nregs = (unsigned)w.regs.size();
+    if (nregs > 0 && w.regs[0].rowperm) {
+        hipLaunchKernelGGL(emicp_order_kernel, dim3(2 * nregs), dim3(kOrdThreads), 0, s,
+                           w.desc.as<EmRegDesc>());
+        EMSA_HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(emicp_setup_kernel, dim3(nregs), dim3(256), 0, s,
                        w.desc.as<EmRegDesc>(), dRt0);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;

@@ -19,6 +19,18 @@ __device__ inline float wave_sum(float v) {
     return v;
 }
 
+// Minimum / maximum over the 64 lanes of a wave; every lane receives it.
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+
 // Sum over the 64 lanes of a wave with DPP-modified adds only (no LDS
 // permutes); the total is valid in lane 63. Within each 16-lane row a
 // butterfly (quad_perm xor 1, xor 2, row_half_mirror, row_mirror) leaves the

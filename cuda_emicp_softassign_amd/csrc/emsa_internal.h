@@ -49,6 +49,7 @@ constexpr int kMomW = 18;      // moments per row block (see emicp.hip)
 constexpr int kEmRows = 64;    // source rows per EM-ICP workgroup (one per lane)
 constexpr int kEmWaves = 4;    // waves per workgroup, splitting the column chunk
 constexpr int kColGroup = 16;  // columns per unrolled group of scalar loads
+constexpr int kCullCols = 8;   // columns per bounding box of the target
 
 struct EmPlan {
     int rowBlocks = 0;   // ceil(M / kEmRows)
@@ -63,8 +64,13 @@ struct EmPlan {
 // contains b.
 struct EmRegDesc {
     const float* X;      // target, SoA [3][N] as given
-    float* Xpad;         // target, SoA [3][ldx], padded with far sentinels
-    const float* Y;      // source, SoA [3][M]
+    float* Xpad;         // target, SoA [3][ldx] in working order, padded with far sentinels
+    const float* Y;      // source, SoA [3][M] as given
+    float* Ys;           // source in working order (an ordered copy, or Y itself)
+    int* rowperm;        // working source row -> caller's row (null: identity)
+    int* colperm;        // working target column -> caller's column (null: identity)
+    float* gbox;         // [ldx / 8][8]: lo xyz, -, hi xyz, - of each 8-column group (null: none)
+    unsigned* sortTmp;   // ordering scratch: 4 N words (target), then 4 M words (source)
     double* state;       // kStateW doubles
     float* partial;      // rowBlocks*chunks*5*kEmRows chunk partial sums
     double* moments;     // kMomW x rowBlocks row-block moments (moment-major)
@@ -90,7 +96,7 @@ struct EmPair {
 
 // Workspace and descriptor table of one EM-ICP batch.
 struct EmWork {
-    DevBuf xpad, partial, moments, state, desc;
+    DevBuf xpad, partial, moments, state, desc, order;
     std::vector<EmRegDesc> regs;
     long long tiles = 0;      // affinity workgroups per step
     long long rowBlocks = 0;  // row blocks per step over the whole table
@@ -133,6 +139,7 @@ struct DeviceCtx {
     bool emPending = false;
     int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
     int emAffinityMfma = 1;    // 1: B and W of the affinity on the matrix cores; 0: on the VALU
+    int emCull = 1;            // 1: skip column groups whose affinities all underflow to 0
     int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)
 };
 
@@ -168,7 +175,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
 
 // Single launches of the setup and affinity kernels (measurement probes).
 int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s);
-int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s);
+// visits (nullable): receives the number of pairs the launch evaluates (the
+// matrix-core kernel; pairs in skipped column groups are not counted).
+int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s,
+                       unsigned long long* visits = nullptr);
 int default_tile_cols(const DeviceCtx& ctx);
 
 // Runs the EM-ICP loop on device buffers for one registration (eager).

@@ -118,6 +118,16 @@ extern "C" int emsa_probe_emicp_affinity(const emsa_pair* pairs, int count, int 
                                 hipMemcpyHostToDevice, s));
     if ((rc = em_launch_setup(ctx->em, ctx->rt.as<float>(), s)) != EMSA_OK) return rc;
     if ((rc = em_launch_affinity(*ctx, ctx->em, sigma_p2, s)) != EMSA_OK) return rc;  // warm-up
+    if (ctx->emAffinityMfma) {  // pairs evaluated by one launch (skipped groups excluded)
+        if ((rc = ctx->misc.reserve(sizeof(unsigned long long))) != EMSA_OK) return rc;
+        unsigned long long* dv = ctx->misc.as<unsigned long long>();
+        EMSA_HIP_TRY(hipMemsetAsync(dv, 0, sizeof(unsigned long long), s));
+        if ((rc = em_launch_affinity(*ctx, ctx->em, sigma_p2, s, dv)) != EMSA_OK) return rc;
+        unsigned long long hv = 0;
+        EMSA_HIP_TRY(hipMemcpyAsync(&hv, dv, sizeof(hv), hipMemcpyDeviceToHost, s));
+        EMSA_HIP_TRY(hipStreamSynchronize(s));
+        np = (double)hv;
+    }
     hipEvent_t e0, e1;
     EMSA_HIP_TRY(hipEventCreate(&e0));
     EMSA_HIP_TRY(hipEventCreate(&e1));
@@ -145,4 +155,15 @@ extern "C" int emsa_probe_set_affinity_path(int use_matrix_cores) {
         ctx->emAffinityMfma = use_matrix_cores ? 1 : 0;
     }
     return emsa_graph_cache_clear();  // recorded sequences hold the previous kernel
+}
+
+extern "C" int emsa_probe_set_affinity_cull(int enable) {
+    clear_error();
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->emCull = enable ? 1 : 0;
+    }
+    return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
 }

@@ -21,7 +21,10 @@ int emsa_probe_rates(double *exp_per_s, double *fma_per_s);
  * (count == 1 && n_starts == 1: the single-registration plan of
  * emsa_emicp_dev), timed with HIP events over `reps` back-to-back launches on
  * the library's stream at the given sigma^2. *pairs_per_launch receives the
- * number of (target, source) pairs one launch evaluates. */
+ * number of (target, source) pairs one launch evaluates: counted on the device
+ * for the matrix-core kernel (column groups skipped because all their
+ * affinities underflow to 0 are not evaluated), N * M per registration for
+ * the all-VALU kernel. */
 struct emsa_pair;
 int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_starts,
                               float sigma_p2, int reps, double *us_per_launch,
@@ -33,6 +36,13 @@ int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_st
  * the same sums up to FP32 rounding; for A/B measurement. Drops the recorded
  * launch sequences of the device. */
 int emsa_probe_set_affinity_path(int use_matrix_cores);
+
+/* Matrix-core EM-ICP affinity of the current device: enable != 0 (the default)
+ * skips, per wave of 64 source rows, the groups of 8 target columns whose
+ * every affinity underflows to exactly 0 in FP32 (bounding boxes of the clouds
+ * in Hilbert order); 0 visits every column. Both give bit-identical results.
+ * For A/B measurement; drops the recorded launch sequences of the device. */
+int emsa_probe_set_affinity_cull(int enable);
 
 #ifdef __cplusplus
 }

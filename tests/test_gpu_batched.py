@@ -158,3 +158,28 @@ def test_affinity_paths_agree(E):
         np.testing.assert_allclose(Rv[k, 0, 9:], Rm[k, 0, 9:], atol=1e-5)
     np.testing.assert_allclose(rv, rm, rtol=1e-4)
     assert geodesic(R1v.astype(float), R1m.astype(float)) < 1e-5
+
+
+def test_affinity_cull_bit_identical(E):
+    """Skipping the target column groups whose affinities all underflow to 0
+    (the default) changes no bit of any result: the per-step traces of a plain
+    registration, and the batched multi-start results and residuals, equal
+    those of the loop over every column in the same (Hilbert) order."""
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    X, Y, _, _ = uniform_cube(3000, seed=21)
+    pairs = _pairs() + [(X, Y)]
+    starts = [np.eye(3), rotation([0, 0, 1], 0.3)]
+    Rt0 = np.stack([E.multistart_rt0(a, b, starts) for a, b in pairs])
+    p = E.registrationParameters()  # full schedule down to sigma^2 = 1e-5
+    try:
+        _lib.check("cull", L.emsa_probe_set_affinity_cull(0))
+        Rf, tf, trf = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p)
+        Rbf, rbf = E.emicp_batched(pairs, Rt0, p)
+    finally:
+        _lib.check("cull", L.emsa_probe_set_affinity_cull(1))
+    Rc, tc, trc = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p)
+    Rbc, rbc = E.emicp_batched(pairs, Rt0, p)
+    assert np.array_equal(trf, trc)
+    assert np.array_equal(Rf, Rc) and np.array_equal(tf, tc)
+    assert np.array_equal(Rbf, Rbc) and np.array_equal(rbf, rbc)
