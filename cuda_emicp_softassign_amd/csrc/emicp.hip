@@ -239,66 +239,80 @@ emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
         __syncthreads();  // this pass's output is the next pass's input
     }
     for (int i = tid; i < n; i += kOrdThreads) perm[i] = (int)vA[i];
+
+    // the cloud in working order: the padded target (far sentinels past N) or
+    // the source copy; 4 points per thread in flight
+    const int ld = src ? n : d.ldx;
+    float* __restrict__ W = src ? d.Ys : d.Xpad;
+    for (int i0 = 0; i0 < ld; i0 += 4 * kOrdThreads) {
+        float v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kOrdThreads + tid;
+            const int o = (i < n) ? (int)vA[i] : -1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[u][a] = (o >= 0) ? P[(size_t)a * n + o] : kSentinel;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * kOrdThreads + tid;
+            if (i < ld)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) W[(size_t)a * ld + i] = v[u][a];
+        }
+    }
+    if (src || !d.gbox) return;
+    __syncthreads();  // the boxes read the padded target written above
+    for (int g = tid; g < d.ldx / kCullCols; g += kOrdThreads) {
+        // empty group (padding only): lo > hi, always skipped; a group with a
+        // non-finite point: the whole space, never skipped
+        float lo3[3] = {INFINITY, INFINITY, INFINITY}, hi3[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool fin = true;
+        const int cn = min(kCullCols, n - g * kCullCols);
+        for (int a = 0; a < 3; ++a) {
+            const float4* q = reinterpret_cast<const float4*>(d.Xpad + (size_t)a * d.ldx + g * kCullCols);
+            const float4 q0 = q[0], q1 = q[1];
+            const float c8[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+            for (int u = 0; u < kCullCols; ++u)
+                if (u < cn) {
+                    fin = fin && fabsf(c8[u]) <= FLT_MAX;
+                    lo3[a] = fminf(lo3[a], c8[u]);
+                    hi3[a] = fmaxf(hi3[a], c8[u]);
+                }
+        }
+        float* b = d.gbox + (size_t)g * 8;
+        *reinterpret_cast<float4*>(b) =
+            make_float4(fin ? lo3[0] : -INFINITY, fin ? lo3[1] : -INFINITY, fin ? lo3[2] : -INFINITY, 0.f);
+        *reinterpret_cast<float4*>(b + 4) =
+            make_float4(fin ? hi3[0] : INFINITY, fin ? hi3[1] : INFINITY, fin ? hi3[2] : INFINITY, 0.f);
+    }
 }
 
-// One workgroup per registration: pad the target in working order (once per
-// target cloud), with the boxes of its column groups and the source in
-// working order; sum the target in FP64 (for floored rows) and load the
-// initial transform.
+// One workgroup per registration: sum the target in FP64 (for floored rows,
+// in the caller's order), pad it when the pair is not reordered (the ordering
+// kernel writes the padded target otherwise; once per target cloud), and load
+// the initial transform.
 __global__ void __launch_bounds__(256)
 emicp_setup_kernel(const EmRegDesc* __restrict__ regs, const float* __restrict__ Rt0) {
     const EmRegDesc& d = regs[blockIdx.x];
-    const int N = d.N, M = d.M, ldx = d.ldx;
+    const int N = d.N, ldx = d.ldx;
+    const bool pad = d.padOwner && !d.colperm;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int c = threadIdx.x; c < ldx; c += blockDim.x) {
         float x = kSentinel, y = kSentinel, z = kSentinel;
         if (c < N) {
-            const int o = d.colperm ? d.colperm[c] : c;
-            x = d.X[o];
-            y = d.X[(size_t)N + o];
-            z = d.X[2 * (size_t)N + o];
+            x = d.X[c];
+            y = d.X[(size_t)N + c];
+            z = d.X[2 * (size_t)N + c];
             sx += x;
             sy += y;
             sz += z;
         }
-        if (d.padOwner) {
+        if (pad) {
             d.Xpad[c] = x;
             d.Xpad[(size_t)ldx + c] = y;
             d.Xpad[2 * (size_t)ldx + c] = z;
-        }
-    }
-    if (d.padOwner && d.rowperm) {
-        for (int r = threadIdx.x; r < M; r += blockDim.x) {
-            const int o = d.rowperm[r];
-            d.Ys[r] = d.Y[o];
-            d.Ys[(size_t)M + r] = d.Y[(size_t)M + o];
-            d.Ys[2 * (size_t)M + r] = d.Y[2 * (size_t)M + o];
-        }
-    }
-    __syncthreads();  // the group boxes read the padded target written above
-    if (d.padOwner && d.gbox) {
-        for (int g = threadIdx.x; g < ldx / kCullCols; g += blockDim.x) {
-            // empty group (padding only): lo > hi, always skipped; a group with a
-            // non-finite point: the whole space, never skipped
-            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            bool fin = true;
-            for (int u = 0; u < kCullCols; ++u) {
-                const int c = g * kCullCols + u;
-                if (c >= N) break;
-                for (int a = 0; a < 3; ++a) {
-                    const float v = d.Xpad[(size_t)a * ldx + c];
-                    fin = fin && fabsf(v) <= FLT_MAX;
-                    lo[a] = fminf(lo[a], v);
-                    hi[a] = fmaxf(hi[a], v);
-                }
-            }
-            float* b = d.gbox + (size_t)g * 8;
-            for (int a = 0; a < 3; ++a) {
-                b[a] = fin ? lo[a] : -INFINITY;
-                b[4 + a] = fin ? hi[a] : INFINITY;
-            }
-            b[3] = 0.f;
-            b[7] = 0.f;
         }
     }
     __shared__ double lds[4 * 3];
@@ -424,177 +438,6 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 #pragma unroll
             for (int w = 1; w < kEmWaves; ++w) acc += red[w][q][lane];
             P[q * kEmRows + lane] = acc;
-        }
-    }
-}
-
-// Stage 1 on the matrix cores: B_r = sum_c e_rc and W_r = sum_c e_rc x_c are
-// a product of the e tile (rows x columns) with [x | y | z | 1] (columns x 4),
-// with the column dimension as the contraction. v_mfma_f32_4x4x1f32 works on
-// 16 independent 4x4 blocks of lanes: lane 4b + j, accumulator i receives
-// a(lane 4b + i) * b(lane 4b + j). With a = e (lane = source row, as in the
-// VALU kernel) and b = component j of the current column, lane 4b + j sums
-// component j (x, y, z, 1) for rows 4b .. 4b + 3. The columns' components are
-// staged per wave in LDS as four rows, so one ds_read_b128 gives a lane its
-// component for 4 columns. Per pair on the VALU: the difference, |d|^2, the
-// scale and C += e^2 (11 ops with the compensation, 8 without) + 1 exp, and
-// one matrix instruction per column and wave. UNR columns per loop iteration,
-// NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
-// 16 and 4 are slower from register pressure). One matrix instruction per
-// column and wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
-// the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
-// cull != 0 (and boxes present): per piece, each of the first pn / 8 lanes
-// tests one 8-column group's box against the box of the wave's valid rows,
-// and the loop visits only the groups with a pair whose exp2 argument can be
-// above -kCullArg (a ballot mask, walked lowest bit first); a piece with no
-// such group is not even staged. visits (nullable, measurement): receives the
-// number of (valid row, valid column) pairs the launch evaluates.
-constexpr int kMfPiece = 256;  // columns staged per wave at a time
-
-template <bool COMP, int UNR, int NACC>
-__global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, int cull,
-                           unsigned long long* __restrict__ visits) {
-    static_assert(UNR == kCullCols, "one loop iteration per column group");
-    static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    const int ri = find_reg(regs, nregs, blockIdx.x);
-    const EmRegDesc& d = regs[ri];
-    const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
-    const int local = blockIdx.x - d.tileBase;
-    const int rb = local / chunks;
-    const int g = local - rb * chunks;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int r = rb * kEmRows + lane;
-
-    float yx = 0.f, yy = 0.f, yz = 0.f;
-    if (r < M) {
-        yx = d.Ys[r];
-        yy = d.Ys[(size_t)M + r];
-        yz = d.Ys[2 * (size_t)M + r];
-    }
-    double pxd, pyd, pzd;
-    transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
-    const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
-    const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
-                plz = (float)(pzd - (double)pz);
-
-    const cfloat_p Xx = (cfloat_p)d.Xpad;
-    const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
-    const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
-    const int cb = (g * kEmWaves + wave) * waveCols;
-    const int ce = min(cb + waveCols, ldx);
-
-    // box of the wave's valid rows; a non-finite row (NaN affinities in every
-    // column) turns the skipping off for the wave
-    const bool rv = r < M;
-    const float nk = -negk;
-    bool doCull = cull && d.gbox;
-    float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
-    if (doCull) {
-        const bool bad = rv && !(fabsf(px) <= FLT_MAX && fabsf(py) <= FLT_MAX && fabsf(pz) <= FLT_MAX);
-        doCull = __ballot(bad) == 0;
-        const float pv[3] = {px, py, pz};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            rlo[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
-                __float_as_int(wave_min(rv ? pv[a] : INFINITY))));
-            rhi[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
-                __float_as_int(wave_max(rv ? pv[a] : -INFINITY))));
-        }
-    }
-
-    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
-    float (*st)[kMfPiece] = stage[wave];
-    const int j = lane & 3;  // this lane's component in the matrix product
-    float C = 0.f;
-    f4 D[NACC];
-#pragma unroll
-    for (int a = 0; a < NACC; ++a) D[a] = f4{0.f, 0.f, 0.f, 0.f};
-    int nvis = 0;  // valid columns visited by this wave
-    for (int p0 = cb; p0 < ce; p0 += kMfPiece) {
-        const int pn = min(kMfPiece, ce - p0);  // a multiple of kColGroup
-        const int ng = pn / kCullCols;
-        unsigned keep = (ng >= 32) ? 0xFFFFFFFFu : ((1u << ng) - 1u);
-        if (doCull) {
-            bool kg = false;
-            if (lane < ng) {
-                const float* bx = d.gbox + (size_t)(p0 / kCullCols + lane) * 8;
-                const float4 lo = *reinterpret_cast<const float4*>(bx);
-                const float4 hi = *reinterpret_cast<const float4*>(bx + 4);
-                const float gx = fmaxf(fmaxf(rlo[0] - hi.x, lo.x - rhi[0]), 0.f);
-                const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
-                const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
-                const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-                kg = !(g2 * nk > kCullArg);
-            }
-            keep = (unsigned)__ballot(kg);
-            if (keep == 0) continue;
-        }
-        // stage [x; y; z; 1] of this piece: 4 consecutive columns per lane
-        for (int c = 4 * lane; c < pn; c += 4 * kWave) {
-            const float4 vx = *reinterpret_cast<const float4*>(d.Xpad + p0 + c);
-            const float4 vy = *reinterpret_cast<const float4*>(d.Xpad + ldx + p0 + c);
-            const float4 vz = *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + p0 + c);
-            *reinterpret_cast<float4*>(&st[0][c]) = vx;
-            *reinterpret_cast<float4*>(&st[1][c]) = vy;
-            *reinterpret_cast<float4*>(&st[2][c]) = vz;
-            *reinterpret_cast<float4*>(&st[3][c]) = make_float4(1.f, 1.f, 1.f, 1.f);
-        }
-        wave_lds_sync();
-        while (keep) {  // wave-uniform: the groups of UNR columns still to visit
-            const int c = __builtin_ctz(keep) * UNR;
-            keep &= keep - 1u;
-            if (visits) nvis += max(0, min(UNR, d.N - (p0 + c)));
-            f4 bq[UNR / 4];
-#pragma unroll
-            for (int v = 0; v < UNR / 4; ++v)
-                bq[v] = *reinterpret_cast<const f4*>(&st[j][c + 4 * v]);
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int cc = p0 + c + u;
-                const float xx = Xx[cc], xy = Xy[cc], xz = Xz[cc];
-                float dx = xx - px, dy = xy - py, dz = xz - pz;
-                if (COMP) {
-                    dx -= plx;
-                    dy -= ply;
-                    dz -= plz;
-                }
-                float s = dx * dx;
-                s = fmaf(dy, dy, s);
-                s = fmaf(dz, dz, s);
-                const float e = __builtin_amdgcn_exp2f(s * negk);
-                C = fmaf(e, e, C);
-                D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[u >> 2][u & 3],
-                                                                 D[u % NACC], 0, 0, 0);
-            }
-        }
-        wave_lds_sync();
-    }
-#pragma unroll
-    for (int a = 1; a < NACC; ++a) D[0] += D[a];
-    if (visits && lane == 0 && nvis > 0)
-        atomicAdd(visits, (unsigned long long)nvis * (unsigned long long)min(kEmRows, M - rb * kEmRows));
-
-    // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
-    // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
-    // (slots 2, 3, 4), j = 3 -> B (slot 1).
-    __shared__ float red[kEmWaves][5][kEmRows];
-    red[wave][0][lane] = C;
-    const int q = (j == 3) ? 1 : 2 + j;
-    const int rb4 = lane & ~3;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) red[wave][q][rb4 + i] = D[0][i];
-    __syncthreads();
-    if (wave == 0) {
-        float* P = d.partial + (size_t)local * 5 * kEmRows;
-#pragma unroll
-        for (int qq = 0; qq < 5; ++qq) {
-            float acc = red[0][qq][lane];
-#pragma unroll
-            for (int w = 1; w < kEmWaves; ++w) acc += red[w][qq][lane];
-            P[qq * kEmRows + lane] = acc;
         }
     }
 }
@@ -760,6 +603,192 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
         o[31] = ok ? 0.0 : 1.0;
     }
 }
+
+// Stage 1 on the matrix cores: B_r = sum_c e_rc and W_r = sum_c e_rc x_c are
+// a product of the e tile (rows x columns) with [x | y | z | 1] (columns x 4),
+// with the column dimension as the contraction. v_mfma_f32_4x4x1f32 works on
+// 16 independent 4x4 blocks of lanes: lane 4b + j, accumulator i receives
+// a(lane 4b + i) * b(lane 4b + j). With a = e (lane = source row, as in the
+// VALU kernel) and b = component j of the current column, lane 4b + j sums
+// component j (x, y, z, 1) for rows 4b .. 4b + 3. The columns' components are
+// staged per wave in LDS as four rows, so one ds_read_b128 gives a lane its
+// component for 4 columns. Per pair on the VALU: the difference, |d|^2, the
+// scale and C += e^2 (11 ops with the compensation, 8 without) + 1 exp, and
+// one matrix instruction per column and wave. UNR columns per loop iteration,
+// NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
+// 16 and 4 are slower from register pressure). One matrix instruction per
+// column and wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
+// the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
+// cull != 0 (and boxes present): per piece of up to 32 groups of kCullCols
+// columns, lane jj tests group jj's box against the box of the wave's valid
+// rows, and the loop visits only the groups with a pair whose exp2 argument
+// can be above -kCullArg (a ballot mask, walked lowest bit first); a piece
+// with no such group is not even staged. visits (nullable, measurement): receives the
+// number of (valid row, valid column) pairs the launch evaluates.
+constexpr int kMfPiece = 256;  // columns staged per wave at a time
+
+template <bool COMP, int UNR, int NACC>
+__global__ void __launch_bounds__(kEmRows * kEmWaves)
+emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, int cull,
+                           unsigned long long* __restrict__ visits) {
+    static_assert(UNR == kCullCols, "one loop iteration per column group");
+    static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int ri = find_reg(regs, nregs, blockIdx.x);
+    const EmRegDesc& d = regs[ri];
+    const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
+    const int local = blockIdx.x - d.tileBase;
+    const int rb = local / chunks;
+    const int g = local - rb * chunks;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int r = rb * kEmRows + lane;
+
+    float yx = 0.f, yy = 0.f, yz = 0.f;
+    if (r < M) {
+        yx = d.Ys[r];
+        yy = d.Ys[(size_t)M + r];
+        yz = d.Ys[2 * (size_t)M + r];
+    }
+    double pxd, pyd, pzd;
+    transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
+    const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
+    const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
+                plz = (float)(pzd - (double)pz);
+
+    const cfloat_p Xx = (cfloat_p)d.Xpad;
+    const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
+    const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
+
+    // box of the wave's valid rows; a non-finite row (NaN affinities in every
+    // column) turns the skipping off for the wave
+    const bool rv = r < M;
+    const float nk = -negk;
+    bool doCull = cull && d.gbox;
+    float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
+    if (doCull) {
+        const bool bad = rv && !(fabsf(px) <= FLT_MAX && fabsf(py) <= FLT_MAX && fabsf(pz) <= FLT_MAX);
+        doCull = __ballot(bad) == 0;
+        const float pv[3] = {px, py, pz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            rlo[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(wave_min(rv ? pv[a] : INFINITY))));
+            rhi[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_int(wave_max(rv ? pv[a] : -INFINITY))));
+        }
+    }
+
+    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
+    float (*st)[kMfPiece] = stage[wave];
+    const int j = lane & 3;  // this lane's component in the matrix product
+    float C = 0.f;
+    f4 D[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) D[a] = f4{0.f, 0.f, 0.f, 0.f};
+    int nvis = 0;  // valid columns visited by this wave
+    // The chunk's groups of kCullCols columns are dealt to the 4 waves in
+    // turn (group G0 + wave + 4k), so that the columns near the row block,
+    // which are consecutive in working order, are shared by all waves.
+    constexpr int kPieceGroups = kMfPiece / kCullCols;
+    const int gpw = waveCols / kCullCols;  // groups per wave
+    const int G0 = g * kEmWaves * gpw;
+    const int Gend = min(G0 + kEmWaves * gpw, ldx / kCullCols);
+    const int nw = (Gend - G0 - wave + kEmWaves - 1) / kEmWaves;  // this wave's groups
+    for (int k0 = 0; k0 < nw; k0 += kPieceGroups) {
+        const int ng = min(kPieceGroups, nw - k0);
+        const int gbase = G0 + wave + kEmWaves * k0;  // group of piece slot jj: gbase + 4 jj
+        unsigned keep = (ng >= 32) ? 0xFFFFFFFFu : ((1u << ng) - 1u);
+        if (doCull) {
+            bool kg = false;
+            if (lane < ng) {
+                const float* bx = d.gbox + (size_t)(gbase + kEmWaves * lane) * 8;
+                const float4 lo = *reinterpret_cast<const float4*>(bx);
+                const float4 hi = *reinterpret_cast<const float4*>(bx + 4);
+                const float gx = fmaxf(fmaxf(rlo[0] - hi.x, lo.x - rhi[0]), 0.f);
+                const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
+                const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
+                const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+                kg = !(g2 * nk > kCullArg);
+            }
+            keep = (unsigned)__ballot(kg);
+            if (keep == 0) continue;
+        }
+        // stage [x; y; z; 1] of the piece: lane 2 jj + h holds columns 4h .. 4h+3
+        // of slot jj
+        {
+            const int jl = lane >> 1, h = lane & 1;
+            if (jl < ng) {
+                const int col = (gbase + kEmWaves * jl) * kCullCols + 4 * h;
+                const int sc = jl * kCullCols + 4 * h;
+                *reinterpret_cast<float4*>(&st[0][sc]) =
+                    *reinterpret_cast<const float4*>(d.Xpad + col);
+                *reinterpret_cast<float4*>(&st[1][sc]) =
+                    *reinterpret_cast<const float4*>(d.Xpad + ldx + col);
+                *reinterpret_cast<float4*>(&st[2][sc]) =
+                    *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + col);
+                *reinterpret_cast<float4*>(&st[3][sc]) = make_float4(1.f, 1.f, 1.f, 1.f);
+            }
+        }
+        wave_lds_sync();
+        while (keep) {  // wave-uniform: the slots still to visit
+            const int jj = __builtin_ctz(keep);
+            keep &= keep - 1u;
+            const int c = jj * UNR;                               // in the stage
+            const int cc0 = (gbase + kEmWaves * jj) * kCullCols;  // in the target
+            if (visits) nvis += max(0, min(UNR, d.N - cc0));
+            f4 bq[UNR / 4];
+#pragma unroll
+            for (int v = 0; v < UNR / 4; ++v)
+                bq[v] = *reinterpret_cast<const f4*>(&st[j][c + 4 * v]);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int cc = cc0 + u;
+                const float xx = Xx[cc], xy = Xy[cc], xz = Xz[cc];
+                float dx = xx - px, dy = xy - py, dz = xz - pz;
+                if (COMP) {
+                    dx -= plx;
+                    dy -= ply;
+                    dz -= plz;
+                }
+                float s = dx * dx;
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                const float e = __builtin_amdgcn_exp2f(s * negk);
+                C = fmaf(e, e, C);
+                D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[u >> 2][u & 3],
+                                                                 D[u % NACC], 0, 0, 0);
+            }
+        }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int a = 1; a < NACC; ++a) D[0] += D[a];
+    if (visits && lane == 0 && nvis > 0)
+        atomicAdd(visits, (unsigned long long)nvis * (unsigned long long)min(kEmRows, M - rb * kEmRows));
+
+    // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
+    // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
+    // (slots 2, 3, 4), j = 3 -> B (slot 1).
+    __shared__ float red[kEmWaves][5][kEmRows];
+    red[wave][0][lane] = C;
+    const int q = (j == 3) ? 1 : 2 + j;
+    const int rb4 = lane & ~3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][q][rb4 + i] = D[0][i];
+    __syncthreads();
+    if (wave == 0) {
+        float* P = d.partial + (size_t)local * 5 * kEmRows;
+#pragma unroll
+        for (int qq = 0; qq < 5; ++qq) {
+            float acc = red[0][qq][lane];
+#pragma unroll
+            for (int w = 1; w < kEmWaves; ++w) acc += red[w][qq][lane];
+            P[qq * kEmRows + lane] = acc;
+        }
+    }
+}
+
 
 // Stage 3: one workgroup per registration.
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
