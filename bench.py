@@ -33,13 +33,16 @@ FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 
 # 1 scale, 1 (C fma); B and W go to the matrix cores (one v_mfma_f32_4x4x1f32
 # per column and wave, i.e. per 64 pairs)
 VALU_PER_PAIR = 11
+VALU_PER_PAIR_PLAIN = 8        # sigma^2 >= 2e-3: plain difference
+COMP_BELOW_SIGMA2 = 2e-3
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
-# (8 x 10k, 8e8 pairs): rocprofv3 --pmc FETCH_SIZE (x2, from a 256 MiB copy
-# calibration in the same process) + WRITE_SIZE, run on `bench.py --counters
-# affinity`; 12.4 MiB fetched + 7.7 MiB written
-AFFINITY_TRAFFIC_BYTES = 21.1e6
+# (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
+# FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
+# WRITE_SIZE, run on `bench.py --counters affinity`; 14.9 MiB fetched (each of
+# the 8 L2s reads the ~1.9 MB of inputs) + 7.9 MiB of partial sums written
+AFFINITY_TRAFFIC_BYTES = 23.9e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
@@ -97,10 +100,45 @@ def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
             "frac_within_1e-4_rad": float(np.mean(errs < 1e-4))}
 
 
+def cull_sweep(_lib, torch):
+    """The batched EM-ICP affinity launch of the 8 x 10k workload (identity
+    transforms) at sigma^2 from 1e-2 down to the schedule's last step, with the
+    column groups whose affinities are all exactly 0 skipped (default) and with
+    every column visited: launch time (HIP events, 10 launches) and the pairs
+    the launch evaluates. One JSON line per sigma^2."""
+    from cuda_emicp_softassign_amd.clouds import uniform_cube
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    keep = []
+    arr = (_lib.emsa_pair * 8)()
+    for k in range(8):
+        X, Y, _, _ = uniform_cube(10000, seed=10 + k)
+        dx = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+        dy = torch.from_numpy(np.ascontiguousarray(Y.T)).to(dev)
+        keep += [dx, dy]
+        arr[k] = _lib.emsa_pair(dx.data_ptr(), 10000, dy.data_ptr(), 10000)
+    try:
+        for s2 in (1e-2, 2e-3, 1e-3, 5e-4, 2e-4, 1e-4, 5e-5, 2e-5, 1.0611e-5):
+            row = {"sigma_p2": s2}
+            for cull, tag in ((1, "skip"), (0, "every_column")):
+                _lib.check("cull", L.emsa_probe_set_affinity_cull(cull))
+                us, npairs = ctypes.c_double(), ctypes.c_double()
+                _lib.check("probe_affinity", L.emsa_probe_emicp_affinity(
+                    arr, 8, 1, ctypes.c_float(s2), 10, ctypes.byref(us), ctypes.byref(npairs)))
+                row[tag + "_us"] = us.value
+                row[tag + "_pairs"] = npairs.value
+            row["frac_pairs_evaluated"] = row["skip_pairs"] / row["every_column_pairs"]
+            row["speedup"] = row["every_column_us"] / row["skip_us"]
+            print(json.dumps(row), flush=True)
+    finally:
+        _lib.check("cull", L.emsa_probe_set_affinity_cull(1))
+
+
 def counters_mode(which, E, _lib, torch):
     """Workloads for rocprofv3 --pmc runs (no hipGraph in the process):
-    affinity -- 20 launches of the batched EM-ICP affinity kernel on the 8 x 10k
-    workload (8e8 pairs each); softassign -- one materialised N = M = 5032
+    affinity -- the 66-step batched EM-ICP sequence of the 8 x 10k workload,
+    launched directly, twice (emsa_probe_emicp_sequence); softassign -- one
+    materialised N = M = 5032
     registration at the reference defaults. Then a 256 MiB device copy of known
     size for the counter scale."""
     from cuda_emicp_softassign_amd.clouds import surface_with_outliers, uniform_cube
@@ -115,10 +153,13 @@ def counters_mode(which, E, _lib, torch):
             dy = torch.from_numpy(np.ascontiguousarray(Y.T)).to(dev)
             keep += [dx, dy]
             arr[k] = _lib.emsa_pair(dx.data_ptr(), 10000, dy.data_ptr(), 10000)
-        us, npairs = ctypes.c_double(), ctypes.c_double()
-        _lib.check("probe_affinity", _lib.load().emsa_probe_emicp_affinity(
-            arr, 8, 1, ctypes.c_float(1e-3), 20, ctypes.byref(us), ctypes.byref(npairs)))
-        out.update(avg_launch_us=us.value, pairs_per_launch=npairs.value)
+        seq_us, seq_pairs, seq_n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        cp = E.registrationParameters().to_c()
+        _lib.check("probe_sequence", _lib.load().emsa_probe_emicp_sequence(
+            arr, 8, 1, ctypes.byref(cp), ctypes.byref(seq_us), ctypes.byref(seq_pairs),
+            ctypes.byref(seq_n), None, 0))
+        out.update(avg_launch_us=seq_us.value / seq_n.value,
+                   pairs_per_launch=seq_pairs.value / seq_n.value, launches=seq_n.value)
     else:
         X, Y, _, _ = surface_with_outliers(5032, seed=1)
         p = E.registrationParameters()
@@ -305,6 +346,9 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the fwd+bwd workloads")
     ap.add_argument("--no-softassign", action="store_true",
                     help="skip the N=M=5032 Softassign timing")
+    ap.add_argument("--cull-sweep", action="store_true",
+                    help="only the EM-ICP affinity of the 8 x 10k batch at a range of sigma^2, "
+                         "with and without skipping zero column groups")
     ap.add_argument("--counters", choices=["affinity", "softassign"], default=None,
                     help="counter-collection mode: run only this kernel's workload with direct "
                          "launches (no graph replay), then a 256 MiB device copy that "
@@ -324,6 +368,9 @@ def main():
 
     if args.counters:
         counters_mode(args.counters, E, _lib, torch)
+        return
+    if args.cull_sweep:
+        cull_sweep(_lib, torch)
         return
 
     n, B = args.n, args.pairs
@@ -398,17 +445,31 @@ def main():
     arr = (_lib.emsa_pair * B)()
     for k, (x, nn, y, mm) in enumerate(dpairs):
         arr[k] = _lib.emsa_pair(x, nn, y, mm)
-    us, npairs = ctypes.c_double(), ctypes.c_double()
-    _lib.check("probe_affinity", _lib.load().emsa_probe_emicp_affinity(
-        arr, B, 1, ctypes.c_float(1e-3), 20, ctypes.byref(us), ctypes.byref(npairs)))
+    # every affinity launch of the workload's own 66-step sequence (launched
+    # directly, HIP events around each launch; evaluated pairs counted on the
+    # device), so the figure covers the skipping at every sigma^2
+    seq_us, seq_pairs, seq_n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    cp = p.to_c()
+    steps_buf = (ctypes.c_double * (3 * n_sigma))()
+    _lib.check("probe_sequence", _lib.load().emsa_probe_emicp_sequence(
+        arr, B, 1, ctypes.byref(cp), ctypes.byref(seq_us), ctypes.byref(seq_pairs),
+        ctypes.byref(seq_n), steps_buf, n_sigma))
+    per_step = np.frombuffer(steps_buf, dtype=np.float64).reshape(n_sigma, 3)
+    us = ctypes.c_double(seq_us.value / seq_n.value)
+    npairs = ctypes.c_double(seq_pairs.value / seq_n.value)
     pe, pf = ctypes.c_double(), ctypes.c_double()
     _lib.check("probe", _lib.load().emsa_probe_rates(ctypes.byref(pe), ctypes.byref(pf)))
     pair_rate = npairs.value / (us.value * 1e-6)
     achieved = pair_rate * FLOPS_PER_PAIR / 1e12
-    # ceilings of the per-pair instruction mix: VALU issue alone (the exp
-    # co-issues, as the measured kernel shows) and VALU + exp fully serialised
-    ceiling = pf.value / VALU_PER_PAIR
-    ceiling_serial = 1.0 / (VALU_PER_PAIR / pf.value + 1.0 / pe.value)
+    # ceilings of the per-pair instruction mix, each step's pairs at its own
+    # VALU count (compensated difference below sigma^2 = 2e-3): VALU issue
+    # alone (the exp co-issues, as the measured kernel shows) and VALU + exp
+    # fully serialised
+    valu = np.where(per_step[:, 0] < COMP_BELOW_SIGMA2, VALU_PER_PAIR, VALU_PER_PAIR_PLAIN)
+    t_issue = float(np.sum(per_step[:, 2] * valu)) / pf.value
+    t_serial = t_issue + float(np.sum(per_step[:, 2])) / pe.value
+    ceiling = seq_pairs.value / t_issue
+    ceiling_serial = seq_pairs.value / t_serial
 
     out = {
         "metric": METRIC,
@@ -450,14 +511,15 @@ def main():
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": AFFINITY_TRAFFIC_BYTES if (n, B) == (10000, 8) else None,
             "avg_launch_us": us.value,
-            "sigma_p2": 1e-3,
+            "launches": seq_n.value,
+            "launches_of": "the %d sigma^2 steps of the workload's sequence" % seq_n.value,
             "pairs_per_launch": npairs.value,
             "pairs_per_launch_all": float(B) * n * n,
             "frac_pairs_evaluated": npairs.value / (float(B) * n * n),
             "algorithmic": "%d FP32 flops + 1 exp per (x, y) pair" % FLOPS_PER_PAIR,
             "exp_rate_per_s": pe.value,
             "fma_rate_per_s": pf.value,
-            "valu_per_pair": VALU_PER_PAIR,
+            "valu_per_pair": float(np.sum(per_step[:, 2] * valu) / np.sum(per_step[:, 2])),
             "matrix_instructions_per_pair": 1.0 / 64,
             "pair_rate_per_s": pair_rate,
             "pair_ceiling_per_s": ceiling,

@@ -263,7 +263,13 @@ int emsa_autotune_emicp(const emsa_pair* pairs, int count, int n_starts, const e
     if (!p || n_starts <= 0) return set_error("autotune: bad arguments"), EMSA_EINVAL;
     std::vector<float> sig = sigma_schedule(*p);
     if (sig.empty()) return set_error("autotune: the schedule runs no sigma^2 step"), EMSA_EINVAL;
-    if (sig.size() > 3) sig.resize(3);
+    // 6 steps spread over the schedule: the large-sigma steps visit nearly
+    // every column, the small-sigma ones skip most (different best tiles)
+    if (sig.size() > 6) {
+        std::vector<float> sub;
+        for (int i = 0; i < 6; ++i) sub.push_back(sig[(size_t)i * (sig.size() - 1) / 5]);
+        sig.swap(sub);
+    }
     DeviceCtx* ctx = current_ctx();
     if (!ctx) return EMSA_ENODEV;
     std::lock_guard<std::mutex> lk(ctx->mu);

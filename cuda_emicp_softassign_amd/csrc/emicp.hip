@@ -79,6 +79,8 @@ __device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, in
 // ---------------------------------------------------------------------------
 constexpr int kOrdThreads = 1024;
 constexpr int kOrdWaves = kOrdThreads / kWave;
+constexpr int kOrdRounds = 4;                        // keys per thread per tile
+constexpr int kOrdKeys = kOrdThreads * kOrdRounds;   // keys per tile
 constexpr int kOrdBits = 10;
 // |exp2 argument| beyond which e is 0 in FP32 (2^-150 already rounds to 0);
 // the margin covers the rounding of the box distance.
@@ -118,9 +120,10 @@ __host__ __device__ inline unsigned hilbert_key(unsigned x, unsigned y, unsigned
 // Writes the permutation (working position -> caller's index): the Hilbert
 // key of every point, then a stable least-significant-digit radix sort of
 // (key, index), 8 bits per pass. A pass histograms the digits, then scatters
-// tiles of kOrdThreads keys in order; within a tile, equal digits are ranked
-// inside a wave with ballots and across waves with per-digit prefix counts in
-// LDS. Non-finite points get the largest key.
+// tiles of kOrdKeys keys in order; within a tile, equal digits are ranked
+// inside a wave with ballots (kOrdRounds rounds of 64 consecutive keys) and
+// across waves with per-digit prefix counts in LDS. Non-finite points get the
+// largest key.
 __global__ void __launch_bounds__(kOrdThreads)
 emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
     const EmRegDesc& d = regs[blockIdx.x >> 1];
@@ -136,7 +139,7 @@ emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
 
     __shared__ float boxs[kOrdWaves][6];
-    __shared__ unsigned base[256], tot[256];
+    __shared__ unsigned base[256], tot[256], wsum[4];
     __shared__ unsigned wcnt[kOrdWaves * 256];
 
     // bounding box of the finite points
@@ -193,30 +196,53 @@ emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
         __syncthreads();
         for (int i = tid; i < n; i += kOrdThreads) atomicAdd(&tot[(ki[i] >> sh) & 255u], 1u);
         __syncthreads();
-        if (tid == 0) {
-            unsigned run = 0;
-            for (int b = 0; b < 256; ++b) {
-                base[b] = run;
-                run += tot[b];
+        {  // exclusive scan of the 256 digit counts by waves 0..3
+            unsigned h = 0, x = 0;
+            if (tid < 256) {
+                h = tot[tid];
+                x = h;
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const unsigned y = __shfl_up(x, o, kWave);
+                    if (lane >= o) x += y;
+                }
+                if (lane == kWave - 1) wsum[wave] = x;
+            }
+            __syncthreads();
+            if (tid < 256) {
+                unsigned off = 0;
+                for (int w = 0; w < wave; ++w) off += wsum[w];
+                base[tid] = off + x - h;
             }
         }
-        for (int t0 = 0; t0 < n; t0 += kOrdThreads) {
-            const int i = t0 + tid;
-            const bool valid = i < n;
-            const unsigned key = valid ? ki[i] : 0u;
-            const unsigned val = valid ? vi[i] : 0u;
-            const unsigned dig = (key >> sh) & 255u;
-            // lanes of this wave holding the same digit
-            unsigned long long same = __ballot(valid);
-            for (int b = 0; b < 8; ++b) {
-                const bool bit = ((dig >> b) & 1u) != 0;
-                const unsigned long long bb = __ballot(bit);
-                same &= bit ? bb : ~bb;
+        // tiles of kOrdKeys keys; wave w ranks keys t0 + 256 w + 64 j + lane in
+        // rounds j = 0..3 (index order), keeping its running count per digit
+        // in wcnt (all lanes read it before the digit's first lane adds the
+        // round's count: one wave's LDS instructions execute in order)
+        for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
+        __syncthreads();
+        for (int t0 = 0; t0 < n; t0 += kOrdKeys) {
+            unsigned key[kOrdRounds], val[kOrdRounds], rank[kOrdRounds];
+#pragma unroll
+            for (int j = 0; j < kOrdRounds; ++j) {
+                const int i = t0 + kWave * (kOrdRounds * wave + j) + lane;
+                const bool valid = i < n;
+                key[j] = valid ? ki[i] : 0u;
+                val[j] = valid ? vi[i] : 0u;
+                const unsigned dig = (key[j] >> sh) & 255u;
+                // lanes of this round holding the same digit
+                unsigned long long same = __ballot(valid);
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = ((dig >> b) & 1u) != 0;
+                    const unsigned long long bb = __ballot(bit);
+                    same &= bit ? bb : ~bb;
+                }
+                const unsigned r = (unsigned)__popcll(same & below);
+                const unsigned pre = valid ? wcnt[wave * 256 + dig] : 0u;
+                wave_lds_sync();
+                if (valid && r == 0) wcnt[wave * 256 + dig] = pre + (unsigned)__popcll(same);
+                wave_lds_sync();
+                rank[j] = pre + r;
             }
-            const unsigned rank = (unsigned)__popcll(same & below);
-            for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
-            __syncthreads();
-            if (valid && rank == 0) wcnt[wave * 256 + dig] = (unsigned)__popcll(same);
             __syncthreads();
             if (tid < 256) {  // per digit: offsets of the waves, then the tile total
                 unsigned run = 0;
@@ -228,13 +254,20 @@ emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
                 tot[tid] = run;
             }
             __syncthreads();
-            if (valid) {
-                const unsigned pos = base[dig] + wcnt[wave * 256 + dig] + rank;
-                ko[pos] = key;
-                vo[pos] = val;
+#pragma unroll
+            for (int j = 0; j < kOrdRounds; ++j) {
+                const int i = t0 + kWave * (kOrdRounds * wave + j) + lane;
+                if (i < n) {
+                    const unsigned dig = (key[j] >> sh) & 255u;
+                    const unsigned pos = base[dig] + wcnt[wave * 256 + dig] + rank[j];
+                    ko[pos] = key[j];
+                    vo[pos] = val[j];
+                }
             }
             __syncthreads();
             if (tid < 256) base[tid] += tot[tid];
+            for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
+            __syncthreads();
         }
         __syncthreads();  // this pass's output is the next pass's input
     }
@@ -798,13 +831,27 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
     const int wave = threadIdx.x / kWave;
     const int rowBlocks = d.rowBlocks;
     __shared__ double mom[kMomW];
-    // moment q: lanes stride over the row blocks, then a fixed-order wave sum
-    for (int q = wave; q < kMomW; q += kEmWaves) {
-        const double* src = d.moments + (size_t)q * rowBlocks;
-        double s = 0.0;
-        for (int b = lane; b < rowBlocks; b += kWave) s += src[b];
-        s = wave_sum(s);
-        if (lane == 0) mom[q] = s;
+    // moment q = wave + 4 k: lanes stride over the row blocks, then a
+    // fixed-order wave sum; the wave's moments are independent chains
+    constexpr int kPer = (kMomW + kEmWaves - 1) / kEmWaves;
+    double acc[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+    for (int b = lane; b < rowBlocks; b += kWave) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = wave + kEmWaves * k;
+            if (q < kMomW) acc[k] += d.moments[(size_t)q * rowBlocks + b];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = wave + kEmWaves * k;
+            if (q < kMomW) mom[q] = acc[k];
+        }
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -977,7 +1024,8 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
 }
 
 int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
-               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s) {
+               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s,
+               const EmStepProbe* probe) {
     const int nregs = (int)w.regs.size();
     const EmRegDesc* desc = w.desc.as<EmRegDesc>();
     int rc = em_launch_setup(w, dRt0, s);
@@ -988,9 +1036,11 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
         const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
+        if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it], s));
         launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig < kCompBelowSigma2, negk, s,
-                        nullptr);
+                        (probe && probe->visits) ? probe->visits + it : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
+        if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
         hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
                            nregs, w.rowBlocks, outlier, last, (int)it);
         EMSA_HIP_TRY(hipGetLastError());

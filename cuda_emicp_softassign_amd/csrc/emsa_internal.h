@@ -170,8 +170,17 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
 
 // Enqueues setup, one launch per sigma^2 step, and the output kernel.
 // dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.
+// Measurement hooks of one enqueued sequence (eager launches only): events
+// (nullable, 2 per sigma^2 step) recorded around each affinity launch, and
+// device counters (nullable, one per sigma^2 step) that the affinity launches
+// add their evaluated pairs to.
+struct EmStepProbe {
+    hipEvent_t* events;
+    unsigned long long* visits;
+};
 int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, double* dResid,
-               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s);
+               const emsa_params& p, const std::vector<float>& sigmas, hipStream_t s,
+               const EmStepProbe* probe = nullptr);
 
 // Single launches of the setup and affinity kernels (measurement probes).
 int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s);

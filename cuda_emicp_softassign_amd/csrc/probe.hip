@@ -167,3 +167,84 @@ extern "C" int emsa_probe_set_affinity_cull(int enable) {
     }
     return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
 }
+
+extern "C" int emsa_probe_emicp_sequence(const emsa_pair* pairs, int count, int n_starts,
+                                         const emsa_params* params, double* affinity_us,
+                                         double* pairs_evaluated, int* launches,
+                                         double* per_step, int per_step_cap) {
+    clear_error();
+    if (!pairs || count <= 0 || n_starts <= 0 || !params || !affinity_us || !pairs_evaluated ||
+        !launches)
+        return set_error("probe_emicp_sequence: bad arguments"), EMSA_EINVAL;
+    if (!finite_params(*params)) return set_error("probe_emicp_sequence: bad parameters"), EMSA_EINVAL;
+    const std::vector<float> sig = sigma_schedule(*params);
+    if (sig.empty()) return set_error("probe_emicp_sequence: empty schedule"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    std::vector<EmPair> pv(count);
+    for (int k = 0; k < count; ++k) {
+        if (!pairs[k].X || !pairs[k].Y || pairs[k].N <= 0 || pairs[k].M <= 0)
+            return set_error("probe_emicp_sequence: null or empty cloud"), EMSA_EINVAL;
+        pv[k] = EmPair{pairs[k].X, pairs[k].N, pairs[k].Y, pairs[k].M};
+    }
+    const bool single = (count == 1 && n_starts == 1);
+    int rc = em_prepare(*ctx, ctx->em, pv.data(), count, n_starts,
+                        single ? 0 : default_tile_cols(*ctx), nullptr, nullptr, s);
+    if (rc != EMSA_OK) return rc;
+    const size_t nregs = (size_t)count * n_starts;
+    if ((rc = ctx->rt.reserve(sizeof(float) * 24 * nregs)) != EMSA_OK) return rc;
+    if ((rc = ctx->misc.reserve(sizeof(unsigned long long) * sig.size())) != EMSA_OK) return rc;
+    std::vector<float> rt0(12 * nregs, 0.f);
+    for (size_t i = 0; i < nregs; ++i) rt0[12 * i] = rt0[12 * i + 4] = rt0[12 * i + 8] = 1.f;
+    float* drt = ctx->rt.as<float>();
+    EMSA_HIP_TRY(hipMemcpyAsync(drt, rt0.data(), sizeof(float) * rt0.size(),
+                                hipMemcpyHostToDevice, s));
+    // pass 1: evaluated pairs of every affinity launch of the sequence
+    unsigned long long* dv = ctx->misc.as<unsigned long long>();
+    EMSA_HIP_TRY(hipMemsetAsync(dv, 0, sizeof(unsigned long long) * sig.size(), s));
+    EmStepProbe count_probe{nullptr, ctx->emAffinityMfma ? dv : nullptr};
+    if ((rc = em_enqueue(*ctx, ctx->em, drt, drt + 12 * nregs, nullptr, *params, sig, s,
+                         &count_probe)) != EMSA_OK)
+        return rc;
+    std::vector<unsigned long long> hv(sig.size(), 0ull);
+    EMSA_HIP_TRY(hipMemcpyAsync(hv.data(), dv, sizeof(unsigned long long) * sig.size(),
+                                hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));
+    double dense = 0.0;  // pairs of one launch that visits every column
+    for (int k = 0; k < count; ++k) dense += (double)pairs[k].N * pairs[k].M * n_starts;
+    // pass 2: the same sequence with events around each affinity launch
+    std::vector<hipEvent_t> ev(2 * sig.size(), nullptr);
+    for (auto& e : ev) EMSA_HIP_TRY(hipEventCreate(&e));
+    EmStepProbe time_probe{ev.data(), nullptr};
+    rc = em_enqueue(*ctx, ctx->em, drt, drt + 12 * nregs, nullptr, *params, sig, s, &time_probe);
+    double us = 0.0, np = 0.0;
+    if (rc == EMSA_OK && hipStreamSynchronize(s) == hipSuccess) {
+        for (size_t i = 0; i < sig.size(); ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) {
+                set_error("probe_emicp_sequence: event timing failed");
+                rc = EMSA_EHIP;
+                break;
+            }
+            const double pi = ctx->emAffinityMfma ? (double)hv[i] : dense;
+            us += 1e3 * (double)ms;
+            np += pi;
+            if (per_step && (int)i < per_step_cap) {
+                per_step[3 * i] = (double)sig[i];
+                per_step[3 * i + 1] = 1e3 * (double)ms;
+                per_step[3 * i + 2] = pi;
+            }
+        }
+    } else if (rc == EMSA_OK) {
+        set_error("probe_emicp_sequence: stream synchronize failed");
+        rc = EMSA_EHIP;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc != EMSA_OK) return rc;
+    *affinity_us = us;
+    *pairs_evaluated = np;
+    *launches = (int)sig.size();
+    return EMSA_OK;
+}

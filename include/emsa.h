@@ -144,7 +144,8 @@ int emsa_emicp_batched_dev(const emsa_pair *pairs, int count, const float *dRt0,
                            void *stream);
 
 /* Autotuner: times the candidate work decompositions on these device buffers
- * (the first sigma^2 steps of p) and keeps the fastest for later calls on the
+ * (6 sigma^2 steps spread over the schedule of p) and keeps the fastest for
+ * later calls on the
  * current device. count == 1 && n_starts == 1 tunes the single-registration
  * plan (workgroups per CU, from 2..32), otherwise the batched column tile width
  * (512..8192). The

@@ -26,6 +26,7 @@ int emsa_probe_rates(double *exp_per_s, double *fma_per_s);
  * affinities underflow to 0 are not evaluated), N * M per registration for
  * the all-VALU kernel. */
 struct emsa_pair;
+struct emsa_params;
 int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_starts,
                               float sigma_p2, int reps, double *us_per_launch,
                               double *pairs_per_launch);
@@ -36,6 +37,19 @@ int emsa_probe_emicp_affinity(const struct emsa_pair *pairs, int count, int n_st
  * the same sums up to FP32 rounding; for A/B measurement. Drops the recorded
  * launch sequences of the device. */
 int emsa_probe_set_affinity_path(int use_matrix_cores);
+
+/* Runs the whole EM-ICP sequence (identity starts, the schedule of *params)
+ * of a batch laid out as emsa_emicp_batched_dev lays it out (count == 1 &&
+ * n_starts == 1: as emsa_emicp_dev), launched directly on the library's stream,
+ * twice: once counting on the device the pairs every affinity launch
+ * evaluates, once with HIP events around each affinity launch. Receives the
+ * summed affinity time in microseconds, the summed evaluated pairs and the
+ * number of affinity launches (sigma^2 steps); per_step (nullable) receives,
+ * for the first per_step_cap steps, (sigma^2, microseconds, pairs) each. */
+int emsa_probe_emicp_sequence(const struct emsa_pair *pairs, int count, int n_starts,
+                              const struct emsa_params *params, double *affinity_us,
+                              double *pairs_evaluated, int *launches, double *per_step,
+                              int per_step_cap);
 
 /* Matrix-core EM-ICP affinity of the current device: enable != 0 (the default)
  * skips, per wave of 64 source rows, the groups of 8 target columns whose
