@@ -802,21 +802,24 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 
     // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
     // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
-    // (slots 2, 3, 4), j = 3 -> B (slot 1).
-    __shared__ float red[kEmWaves][5][kEmRows];
-    red[wave][0][lane] = C;
+    // (slots 2, 3, 4), j = 3 -> B (slot 1). Each wave's 5 x 64 partials go in
+    // its own stage, which it no longer reads (its LDS accesses are in order),
+    // so the workgroup needs 16 KB of LDS and 8 waves per SIMD fit.
+    static_assert(5 * kEmRows <= 4 * kMfPiece, "partials fit a wave's stage");
+    float* red = &stage[wave][0][0];
+    red[lane] = C;
     const int q = (j == 3) ? 1 : 2 + j;
     const int rb4 = lane & ~3;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) red[wave][q][rb4 + i] = D[0][i];
+    for (int i = 0; i < 4; ++i) red[q * kEmRows + rb4 + i] = D[0][i];
     __syncthreads();
     if (wave == 0) {
         float* P = d.partial + (size_t)local * 5 * kEmRows;
 #pragma unroll
         for (int qq = 0; qq < 5; ++qq) {
-            float acc = red[0][qq][lane];
+            float acc = stage[0][0][qq * kEmRows + lane];
 #pragma unroll
-            for (int w = 1; w < kEmWaves; ++w) acc += red[w][qq][lane];
+            for (int w = 1; w < kEmWaves; ++w) acc += stage[w][0][qq * kEmRows + lane];
             P[qq * kEmRows + lane] = acc;
         }
     }
