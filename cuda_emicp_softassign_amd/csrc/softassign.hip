@@ -25,8 +25,10 @@
 // Launches per inner iteration (I1 = 3: 11), with no signalling between
 // workgroups: row passes give each wave whole rows (its sums need no other
 // workgroup); a column pass is a partial-sum kernel over row chunks plus a
-// finalize kernel per 256 columns; the final row pass writes per-wave FP64
-// moments that one solve workgroup reduces (fixed order) before Horn's step.
+// finalize kernel per 64 columns; the final row pass writes per-workgroup
+// FP64 moments that one solve workgroup reduces (fixed order) before Horn's
+// step, and the next inner iteration's first row pass transforms the sources
+// with the new (R, t) as it streams them.
 // S is assembled from FP64 moments of the final pass:
 //   D_r = D'_r - s_r Yc with D'_r = sum_c M_rc b_c y_c, and
 //   S_ij = sum s x_i D'_j - Yc_j sum s^2 x_i - Xc_i sum D'_j + Xc_i Yc_j Sigma.
@@ -43,7 +45,8 @@ namespace {
 constexpr int kSaWaves = 4;                  // waves per workgroup
 constexpr int kSaRowsWG = 8;                 // rows per row-pass workgroup (16: too few workgroups)
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
-constexpr int kSaMomW = 22;                  // per-wave moments of the final pass
+constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
+constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
 constexpr float kSaSentinel = 1.0e15f;
 
 struct SaPlan {
@@ -70,7 +73,8 @@ SaPlan plan_sa(int N, int M, int numCU) {
 struct SaBufs {
     const float* X;   // [3][N]
     const float* Y;   // [3][M]
-    float* P;         // [3][ldk] transformed source points R y + t
+    float* P;         // [3][ldk] transformed source points R y + t (written by the first row pass)
+    float* Yp;        // [3][ldk] source points, zero past M (aligned vector loads)
     float* K;         // [N][ldk] (materialised only)
     float* u;         // [N] row scale (log: f = log2 u)
     float* v;         // [ldk] column scale (log: g = log2 v)
@@ -79,40 +83,20 @@ struct SaBufs {
     float* bcol;      // [ldk] last column sums b_c
     float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
     float* cpart;     // [H][ldk] column partial sums
-    double* rmom;     // [rowBlocks][kSaMomW] per-workgroup moments of the final pass
-    double* cby;      // [ldk / 64][3] sum_c b_c y_c per finalize block
+    double* rmom;     // [kSaMomPad][rowBlocks] per-workgroup moments of the final pass
+    double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
     double* trace;    // nullable
 };
 
-// Transform the source points with (R, t) into P; columns >= M get the
-// sentinel (their affinities underflow to exactly 0).
-__device__ inline void sa_transform(const SaBufs& b, int M, int ldk, const double* R,
-                                    const double* t, int tid, int nthreads) {
-    for (int c = tid; c < ldk; c += nthreads) {
-        float px = kSaSentinel, py = kSaSentinel, pz = kSaSentinel;
-        if (c < M) {
-            const double y0 = b.Y[c], y1 = b.Y[(size_t)M + c], y2 = b.Y[2 * (size_t)M + c];
-            px = (float)(R[0] * y0 + R[1] * y1 + R[2] * y2 + t[0]);
-            py = (float)(R[3] * y0 + R[4] * y1 + R[5] * y2 + t[1]);
-            pz = (float)(R[6] * y0 + R[7] * y1 + R[8] * y2 + t[2]);
-        }
-        b.P[c] = px;
-        b.P[(size_t)ldk + c] = py;
-        b.P[2 * (size_t)ldk + c] = pz;
-    }
-}
-
 __global__ void __launch_bounds__(1024)
 sa_setup_kernel(SaBufs b, int N, int M, int ldk, const float* __restrict__ Rt, float moutlier) {
-    double R[9], t[3];
-    for (int i = 0; i < 9; ++i) R[i] = Rt[i];
-    for (int i = 0; i < 3; ++i) t[i] = Rt[9 + i];
     if (threadIdx.x < 12) b.state[threadIdx.x] = (double)Rt[threadIdx.x];
     if (threadIdx.x == 12) b.state[kStStatus] = 0.0;
     for (int r = threadIdx.x; r < N; r += blockDim.x) b.orow[r] = moutlier;
     for (int c = threadIdx.x; c < M; c += blockDim.x) b.ocol[c] = moutlier;
-    sa_transform(b, M, ldk, R, t, threadIdx.x, blockDim.x);
+    for (int c = threadIdx.x; c < ldk; c += blockDim.x)
+        for (int a = 0; a < 3; ++a) b.Yp[(size_t)a * ldk + c] = (c < M) ? b.Y[(size_t)a * M + c] : 0.f;
 }
 
 __global__ void sa_output_kernel(const double* __restrict__ state, float* __restrict__ Rt) {
@@ -160,12 +144,42 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     for (int j = 0; j < R; ++j)
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[j][q] = 0.f;
+    // MODE 0: the transformed sources P = R y + t of this inner iteration are
+    // formed here (FP64, rounded to FP32) from the state the last solve left;
+    // workgroup 0 stores them for the later passes
+    double Rm[9], tm[3];
+    if (MODE == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Rm[i] = b.state[kStR + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tm[i] = b.state[kStT + i];
+    }
 
     for (int c = 4 * lane + 4 * kWave * wave; c < ldk; c += 4 * kWave * kSaWaves) {
         float w[4][Q];  // per-column weights
         float p[3][4];
         float kg[4];
-        if (LOG || MODE == 0) {
+        if (MODE == 0) {
+            float y[3][4];
+            load4(b.Yp + c, y[0]);
+            load4(b.Yp + (size_t)ldk + c, y[1]);
+            load4(b.Yp + 2 * (size_t)ldk + c, y[2]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double y0 = y[0][i], y1 = y[1][i], y2 = y[2][i];
+                const bool in = c + i < M;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    p[a][i] = in ? (float)(Rm[3 * a] * y0 + Rm[3 * a + 1] * y1 + Rm[3 * a + 2] * y2 +
+                                           tm[a])
+                                 : kSaSentinel;
+            }
+            if (blockIdx.x == 0)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    *reinterpret_cast<float4*>(b.P + (size_t)a * ldk + c) =
+                        make_float4(p[a][0], p[a][1], p[a][2], p[a][3]);
+        } else if (LOG) {
             load4(b.P + c, p[0]);
             load4(b.P + (size_t)ldk + c, p[1]);
             load4(b.P + 2 * (size_t)ldk + c, p[2]);
@@ -269,7 +283,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             for (int jj = 0; jj < 3; ++jj) m[4 + 3 * i + jj] += s * x[i] * D[jj];
         }
     }
-    for (int q = 0; q < kSaMomW; ++q) b.rmom[(size_t)blockIdx.x * kSaMomW + q] = m[q];
+    for (int q = 0; q < kSaMomW; ++q) b.rmom[(size_t)q * gridDim.x + blockIdx.x] = m[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -380,33 +394,44 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) by[i] = wave_sum(by[i]);
-    if (lane < 3) b.cby[(size_t)blockIdx.x * 3 + lane] = (lane == 0) ? by[0] : (lane == 1 ? by[1] : by[2]);
+    if (lane < 3)
+        b.cby[(size_t)lane * gridDim.x + blockIdx.x] = (lane == 0) ? by[0] : (lane == 1 ? by[1] : by[2]);
 }
 
-// One workgroup: totals of the per-wave moments and of sum b y (fixed order),
-// S, Horn, the new (R, t), and the transformed sources of the next iteration.
+// One workgroup: totals of the per-workgroup moments and of sum b y (fixed
+// order: wave w sums quantities w, w + 4, ..., its lanes striding over the
+// partials, then a wave sum), S, Horn and the new (R, t) in the state. The
+// next inner iteration's first row pass transforms the sources with it.
 __global__ void __launch_bounds__(kWave * kSaWaves)
-sa_solve_kernel(SaBufs b, int M, int ldk, int nmom, int nfin, float T, int iter) {
+sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     const int tid = threadIdx.x;
-    double m[kSaMomW];
+    const int lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    // moments q = wave + 4 k (rows up to kSaMomPad - 1 exist, the ones past
+    // kSaMomW are not used): loads issued together, no per-quantity branch
+    constexpr int kPer = kSaMomPad / kSaWaves;
+    double acc[kPer];
 #pragma unroll
-    for (int q = 0; q < kSaMomW; ++q) m[q] = 0.0;
-    for (int w = tid; w < nmom; w += blockDim.x)
+    for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+    for (int w = lane; w < nmom; w += kWave)
 #pragma unroll
-        for (int q = 0; q < kSaMomW; ++q) m[q] += b.rmom[(size_t)w * kSaMomW + q];
-    double by[3] = {0.0, 0.0, 0.0};
-    for (int k = tid; k < nfin; k += blockDim.x)
+        for (int k = 0; k < kPer; ++k) acc[k] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w];
+    double by = 0.0;  // waves 0..2: component wave of sum b y
+    if (wave < 3)
+        for (int w = lane; w < nfin; w += kWave) by += b.cby[(size_t)wave * nfin + w];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) by[i] += b.cby[(size_t)k * 3 + i];
-    __shared__ double lds[kSaWaves * kSaMomW];
+    for (int k = 0; k < kPer; ++k) acc[k] = wave_sum(acc[k]);
+    by = wave_sum(by);
     __shared__ double mt[kSaMomW + 3];
-    double total;
-    block_sum<kSaMomW>(m, lds, kSaWaves, total);
-    if (tid < kSaMomW) mt[tid] = total;
-    block_sum<3>(by, lds, kSaWaves, total);
-    if (tid < 3) mt[kSaMomW + tid] = total;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = wave + kSaWaves * k;
+            if (q < kSaMomW) mt[q] = acc[k];
+        }
+        if (wave < 3) mt[kSaMomW + wave] = by;
+    }
     __syncthreads();
-    __shared__ double newRt[12];
     if (tid == 0) {
         const double Sg = mt[0];
         double Xc[3], Yc[3], S[9], Rn[9], tn[3];
@@ -428,8 +453,6 @@ sa_solve_kernel(SaBufs b, int M, int ldk, int nmom, int nfin, float T, int iter)
             for (int i = 0; i < 9; ++i) Rn[i] = b.state[kStR + i];
             for (int i = 0; i < 3; ++i) tn[i] = b.state[kStT + i];
         }
-        for (int i = 0; i < 9; ++i) newRt[i] = Rn[i];
-        for (int i = 0; i < 3; ++i) newRt[9 + i] = tn[i];
         if (b.trace) {
             double* o = b.trace + (size_t)iter * EMSA_TRACE_W;
             o[0] = T;
@@ -446,8 +469,6 @@ sa_solve_kernel(SaBufs b, int M, int ldk, int nmom, int nfin, float T, int iter)
             o[31] = ok ? 0.0 : 1.0;
         }
     }
-    __syncthreads();
-    sa_transform(b, M, ldk, newRt, newRt + 9, tid, blockDim.x);
 }
 
 }  // namespace
@@ -463,8 +484,9 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     SaBufs b{};
     b.X = dX;
     b.Y = dY;
-    if ((rc = ctx.u.reserve(sizeof(float) * 3 * (size_t)pl.ldk)) != EMSA_OK) return rc;  // P
+    if ((rc = ctx.u.reserve(sizeof(float) * 6 * (size_t)pl.ldk)) != EMSA_OK) return rc;  // P, Yp
     b.P = ctx.u.as<float>();
+    b.Yp = b.P + 3 * (size_t)pl.ldk;
     if (!LOGV) {
         if ((rc = ctx.K.reserve(sizeof(float) * (size_t)N * pl.ldk)) != EMSA_OK) return rc;
         b.K = ctx.K.as<float>();
@@ -482,10 +504,10 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.colpart.reserve(sizeof(float) * (size_t)pl.H * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
-                                ((size_t)nmom * kSaMomW + (size_t)finBlocks * 3))) != EMSA_OK)
+                                ((size_t)nmom * kSaMomPad + (size_t)finBlocks * 3))) != EMSA_OK)
         return rc;
     b.rmom = ctx.samom.as<double>();
-    b.cby = b.rmom + (size_t)nmom * kSaMomW;
+    b.cby = b.rmom + (size_t)nmom * kSaMomPad;
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
@@ -544,8 +566,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
                                    k1, k0);
-            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), blk, 0, s, b, M, pl.ldk, nmom,
-                               finBlocks, T, it);
+            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), blk, 0, s, b, nmom, finBlocks, T, it);
             EMSA_HIP_TRY(hipGetLastError());
         }
         T = T * p.TFACTOR;  // softassign.cu:820, float
