@@ -831,21 +831,20 @@ __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
     const EmRegDesc& d = regs[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int rowBlocks = d.rowBlocks;
     __shared__ double mom[kMomW];
     // moment q = wave + 4 k: lanes stride over the row blocks, then a
-    // fixed-order wave sum; the wave's moments are independent chains
-    constexpr int kPer = (kMomW + kEmWaves - 1) / kEmWaves;
+    // fixed-order wave sum; the wave's loads are issued together (rows past
+    // kMomW exist and are ignored)
+    constexpr int kPer = kMomPad / kEmWaves;
     double acc[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
     for (int b = lane; b < rowBlocks; b += kWave) {
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int q = wave + kEmWaves * k;
-            if (q < kMomW) acc[k] += d.moments[(size_t)q * rowBlocks + b];
-        }
+        for (int k = 0; k < kPer; ++k)
+            acc[k] += d.moments[(size_t)(wave + kEmWaves * k) * rowBlocks + b];
     }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) acc[k] = wave_sum(acc[k]);
@@ -973,7 +972,7 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
             d.partial = reinterpret_cast<float*>(partF);
             partF += (size_t)t * 5 * kEmRows;
             d.moments = reinterpret_cast<double*>(momD);
-            momD += (size_t)pl.rowBlocks * kMomW;
+            momD += (size_t)pl.rowBlocks * kMomPad;
             d.trace = (count == 1 && n_starts == 1) ? trace : nullptr;
             d.rows = (count == 1 && n_starts == 1) ? rows : nullptr;
             d.ckpt = nullptr;

@@ -46,6 +46,7 @@ struct DevBuf {
 constexpr int kStateW = 32;    // R[9] t[3] Xsum[3] status ...
 constexpr int kStR = 0, kStT = 9, kStXsum = 12, kStStatus = 15, kStResid = 16, kStLambda = 17;
 constexpr int kMomW = 18;      // moments per row block (see emicp.hip)
+constexpr int kMomPad = 20;    // moment rows allocated (a multiple of kEmWaves)
 constexpr int kEmRows = 64;    // source rows per EM-ICP workgroup (one per lane)
 constexpr int kEmWaves = 4;    // waves per workgroup, splitting the column chunk
 constexpr int kColGroup = 16;  // columns per unrolled group of scalar loads
@@ -73,7 +74,8 @@ struct EmRegDesc {
     unsigned* sortTmp;   // ordering scratch: 4 N words (target), then 4 M words (source)
     double* state;       // kStateW doubles
     float* partial;      // rowBlocks*chunks*5*kEmRows chunk partial sums
-    double* moments;     // kMomW x rowBlocks row-block moments (moment-major)
+    double* moments;     // kMomPad x rowBlocks row-block moments (moment-major; rows
+                         // kMomW.. are not written or used)
     double* trace;       // nullable: EMSA_TRACE_W doubles per sigma step
     double* rows;        // nullable: C_r, lambda_r (2*M) of the last step
     double* ckpt;        // nullable: [steps][M][5] C_r, lambda_r, lambda_r X'_r per step
