@@ -29,11 +29,13 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 1 (B), 6 (W fma)
-# issued per pair on the VALU at sigma^2 < 2e-3: 6 sub (compensated), 3 (d^2),
-# 1 scale, 1 (C fma); B and W go to the matrix cores (one v_mfma_f32_4x4x1f32
+# issued per pair on the VALU at sigma^2 < 2e-3: 3 fma + 3 sub (the scaled,
+# compensated difference), 3 (d^2), 1 (C fma); the exponent scale rides on the
+# coordinates and the sign on the exp's source modifier; B and W go to the
+# matrix cores (one v_mfma_f32_4x4x1f32
 # per column and wave, i.e. per 64 pairs)
-VALU_PER_PAIR = 11
-VALU_PER_PAIR_PLAIN = 8        # sigma^2 >= 2e-3: plain difference
+VALU_PER_PAIR = 10
+VALU_PER_PAIR_PLAIN = 7        # sigma^2 >= 2e-3: plain difference
 COMP_BELOW_SIGMA2 = 2e-3
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E

@@ -14,8 +14,9 @@
 //   C_r = sum e^2,  B_r = sum e,  W_r = sum e x_c
 // and then lambda_r = B_r / sqrt(C_r), lambda_r X'_r = W_r / sqrt(C_r).
 // Target points are wave-uniform, so they are read with scalar loads and
-// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair, or 11
-// when B and W are summed on the matrix cores (the default kernel).
+// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair, or 10
+// when B and W are summed on the matrix cores and the exponent scale is
+// carried by the coordinates (the default kernel).
 //
 // Work decomposition per sigma^2 step, three launches with no signalling
 // between workgroups (kernel boundaries order them):
@@ -645,8 +646,10 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // VALU kernel) and b = component j of the current column, lane 4b + j sums
 // component j (x, y, z, 1) for rows 4b .. 4b + 3. The columns' components are
 // staged per wave in LDS as four rows, so one ds_read_b128 gives a lane its
-// component for 4 columns. Per pair on the VALU: the difference, |d|^2, the
-// scale and C += e^2 (11 ops with the compensation, 8 without) + 1 exp, and
+// component for 4 columns. Per pair on the VALU: the scaled difference
+// d' = ka x - p' (one FMA per coordinate, plus 3 subtractions for the
+// compensation), |d'|^2 and C += e^2 (10 ops with the compensation, 7
+// without) + 1 exp of -|d'|^2 (the sign is a source modifier), and
 // one matrix instruction per column and wave. UNR columns per loop iteration,
 // NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
 // 16 and 4 are slower from register pressure). One matrix instruction per
@@ -662,7 +665,7 @@ constexpr int kMfPiece = 256;  // columns staged per wave at a time
 
 template <bool COMP, int UNR, int NACC>
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk, int cull,
+emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
                            unsigned long long* __restrict__ visits) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
@@ -686,8 +689,15 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     double pxd, pyd, pzd;
     transform_row(d.state, yx, yy, yz, pxd, pyd, pzd);
     const float px = (float)pxd, py = (float)pyd, pz = (float)pzd;
-    const float plx = (float)(pxd - (double)px), ply = (float)(pyd - (double)py),
-                plz = (float)(pzd - (double)pz);
+    // the row in units of the exponent scale: p' = ka p (FP64, then split
+    // into a rounded part and its residual for the compensated difference),
+    // so that d' = ka x - p' comes from one FMA per coordinate and the
+    // exponent is -|d'|^2 with no scaling multiply
+    const double kad = (double)ka;
+    const double qxd = kad * pxd, qyd = kad * pyd, qzd = kad * pzd;
+    const float qx = (float)qxd, qy = (float)qyd, qz = (float)qzd;
+    const float qlx = (float)(qxd - (double)qx), qly = (float)(qyd - (double)qy),
+                qlz = (float)(qzd - (double)qz);
 
     const cfloat_p Xx = (cfloat_p)d.Xpad;
     const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
@@ -696,7 +706,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     // box of the wave's valid rows; a non-finite row (NaN affinities in every
     // column) turns the skipping off for the wave
     const bool rv = r < M;
-    const float nk = -negk;
+    const float nk = ka * ka;
     bool doCull = cull && d.gbox;
     float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
     if (doCull) {
@@ -777,17 +787,17 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 const int cc = cc0 + u;
-                const float xx = Xx[cc], xy = Xy[cc], xz = Xz[cc];
-                float dx = xx - px, dy = xy - py, dz = xz - pz;
+                float dx = fmaf(Xx[cc], ka, -qx), dy = fmaf(Xy[cc], ka, -qy),
+                      dz = fmaf(Xz[cc], ka, -qz);
                 if (COMP) {
-                    dx -= plx;
-                    dy -= ply;
-                    dz -= plz;
+                    dx -= qlx;
+                    dy -= qly;
+                    dz -= qlz;
                 }
                 float s = dx * dx;
                 s = fmaf(dy, dy, s);
                 s = fmaf(dz, dz, s);
-                const float e = __builtin_amdgcn_exp2f(s * negk);
+                const float e = __builtin_amdgcn_exp2f(-s);
                 C = fmaf(e, e, C);
                 D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[u >> 2][u & 3],
                                                                  D[u % NACC], 0, 0, 0);
@@ -867,16 +877,20 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
 // kCompBelowSigma2; B and W on the matrix cores unless the context selects
 // the all-VALU kernel.
 void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
-                     bool comp, float negk, hipStream_t s, unsigned long long* visits) {
+                     float sigma_p2, hipStream_t s, unsigned long long* visits) {
     const dim3 blk(kEmRows * kEmWaves);
+    const bool comp = sigma_p2 < kCompBelowSigma2;
     if (ctx.emAffinityMfma) {
+        // exponent scale sqrt(log2 e / (2 sigma^2)), applied to the coordinates
+        const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
         if (comp)
             hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk,
-                               0, s, desc, nregs, negk, ctx.emCull, visits);
+                               0, s, desc, nregs, ka, ctx.emCull, visits);
         else
             hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk,
-                               0, s, desc, nregs, negk, ctx.emCull, visits);
+                               0, s, desc, nregs, ka, ctx.emCull, visits);
     } else {
+        const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
         if (comp)
             hipLaunchKernelGGL(emicp_affinity_kernel<true>, dim3(tiles), blk, 0, s, desc, nregs,
                                negk);
@@ -1035,11 +1049,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
-        const float negk = (float)(-kLog2e / (2.0 * (double)sig));
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it], s));
-        launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig < kCompBelowSigma2, negk, s,
+        launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig, s,
                         (probe && probe->visits) ? probe->visits + it : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
@@ -1061,9 +1074,8 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
 
 int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s,
                        unsigned long long* visits) {
-    const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
     launch_affinity(ctx, w.desc.as<EmRegDesc>(), (int)w.regs.size(), (unsigned)w.tiles,
-                    sigma_p2 < kCompBelowSigma2, negk, s, visits);
+                    sigma_p2, s, visits);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
 }
