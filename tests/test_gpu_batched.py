@@ -195,3 +195,38 @@ def test_affinity_cull_bit_identical(E):
     assert np.array_equal(trf, trc)
     assert np.array_equal(Rf, Rc) and np.array_equal(tf, tc)
     assert np.array_equal(Rbf, Rbc) and np.array_equal(rbf, rbc)
+
+
+def test_affinity_cull_non_finite_inputs(E):
+    """The device entry point takes clouds as they are. A NaN in the target
+    makes its column group's box the whole space and a NaN in the source
+    turns the skipping off for its wave, so a non-finite input propagates
+    exactly as in the loop over every column: same outputs bit for bit (NaN
+    included), with the skipping on and off."""
+    import torch
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    X, Y, _, _ = uniform_cube(1500, seed=31)
+    p = E.registrationParameters(sigma_inf=1e-4)
+    stream = torch.cuda.current_stream().cuda_stream
+    for where in ("target", "source", "target-inf"):
+        Xb, Yb = X.copy(), Y.copy()
+        if where == "target":
+            Xb[700, 2] = np.nan
+        elif where == "source":
+            Yb[40, 0] = np.nan
+        else:
+            Xb[10, 1] = np.inf
+        dX = torch.from_numpy(np.ascontiguousarray(Xb.T)).cuda()
+        dY = torch.from_numpy(np.ascontiguousarray(Yb.T)).cuda()
+        res = []
+        for cull in (0, 1):
+            _lib.check("cull", L.emsa_probe_set_affinity_cull(cull))
+            dRt = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float32,
+                               device="cuda")
+            E.emicp_dev(dX.data_ptr(), len(Xb), dY.data_ptr(), len(Yb), dRt.data_ptr(), p,
+                        stream)
+            torch.cuda.synchronize()
+            res.append(dRt.cpu().numpy())
+        _lib.check("cull", L.emsa_probe_set_affinity_cull(1))
+        assert np.array_equal(res[0], res[1], equal_nan=True), (where, res)
