@@ -651,9 +651,10 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // compensation), |d'|^2 and C += e^2 (10 ops with the compensation, 7
 // without) + 1 exp of -|d'|^2 (the sign is a source modifier), and
 // one matrix instruction per column and wave. UNR columns per loop iteration,
-// NACC independent accumulators (measured on MI355X: 8 and 2 are the fastest;
-// 16 and 4 are slower from register pressure). One matrix instruction per
-// column and wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
+// NACC independent accumulators (measured on MI355X, 8 x 10k batched
+// sequence: NACC 4 is 4% faster than 2 and 1, 8 is 8% slower; UNR is the
+// column group of the skipping, 8). One matrix instruction per column and
+// wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
 // the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
 // cull != 0 (and boxes present): per piece of up to 32 groups of kCullCols
 // columns, lane jj tests group jj's box against the box of the wave's valid
@@ -884,10 +885,10 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
         // exponent scale sqrt(log2 e / (2 sigma^2)), applied to the coordinates
         const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
         if (comp)
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 2>), dim3(tiles), blk,
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 4>), dim3(tiles), blk,
                                0, s, desc, nregs, ka, ctx.emCull, visits);
         else
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 2>), dim3(tiles), blk,
+            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 4>), dim3(tiles), blk,
                                0, s, desc, nregs, ka, ctx.emCull, visits);
     } else {
         const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
