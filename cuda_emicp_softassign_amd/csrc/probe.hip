@@ -248,3 +248,25 @@ extern "C" int emsa_probe_emicp_sequence(const emsa_pair* pairs, int count, int 
     *launches = (int)sig.size();
     return EMSA_OK;
 }
+
+extern "C" int emsa_probe_hilbert_order(const float* dP, int n, int* dPerm) {
+    clear_error();
+    if (!dP || !dPerm || n <= 0) return set_error("probe_hilbert_order: bad arguments"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    // one registration with the cloud as target and source: the ordering
+    // kernel writes the target permutation
+    EmPair pr{dP, n, dP, n};
+    int rc = em_prepare(*ctx, ctx->em, &pr, 1, 1, 0, nullptr, nullptr, s);
+    if (rc != EMSA_OK) return rc;
+    if ((rc = ctx->rt.reserve(sizeof(float) * 12)) != EMSA_OK) return rc;
+    const float rt0[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    EMSA_HIP_TRY(hipMemcpyAsync(ctx->rt.ptr, rt0, sizeof(rt0), hipMemcpyHostToDevice, s));
+    if ((rc = em_launch_setup(ctx->em, ctx->rt.as<float>(), s)) != EMSA_OK) return rc;
+    EMSA_HIP_TRY(hipMemcpyAsync(dPerm, ctx->em.regs[0].colperm, sizeof(int) * (size_t)n,
+                                hipMemcpyDeviceToDevice, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));
+    return EMSA_OK;
+}

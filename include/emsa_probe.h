@@ -51,6 +51,13 @@ int emsa_probe_emicp_sequence(const struct emsa_pair *pairs, int count, int n_st
                               double *pairs_evaluated, int *launches, double *per_step,
                               int per_step_cap);
 
+/* The working order EM-ICP gives a cloud: dP is a device SoA cloud [3][n];
+ * dPerm (device, n ints) receives the permutation (working position ->
+ * caller's index): points sorted stably by their 30-bit Hilbert key over the
+ * cloud's bounding box (10 bits per axis), non-finite points last. For
+ * tests of the ordering. */
+int emsa_probe_hilbert_order(const float *dP, int n, int *dPerm);
+
 /* Matrix-core EM-ICP affinity of the current device: enable != 0 (the default)
  * skips, per wave of 64 source rows, the groups of 8 target columns whose
  * every affinity underflows to exactly 0 in FP32 (bounding boxes of the clouds
