@@ -43,7 +43,7 @@ namespace emsa {
 namespace {
 
 constexpr int kSaWaves = 4;                  // waves per workgroup
-constexpr int kSaRowsWG = 8;                 // rows per row-pass workgroup (16: too few workgroups)
+constexpr int kSaRowsWG = 4;                 // rows per row-pass workgroup (4 measured faster than 8, 16)
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
 constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
 constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
@@ -261,13 +261,12 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         return;
     }
 
-    // MODE 2: FP64 moments of this workgroup's rows, in row order (thread 0)
+    // MODE 2: FP64 moments of this workgroup's rows: lane j < nr forms row
+    // j's terms, then each moment is summed over the rows in row order
     //  [0] s  [1..3] s x  [4 + 3i + j] s x_i D'_j  [13..15] s^2 x  [16..18] D'
-    if (threadIdx.x != 0) return;
-    double m[kSaMomW];
-#pragma unroll
-    for (int q = 0; q < kSaMomW; ++q) m[q] = 0.0;
-    for (int j = 0; j < nr; ++j) {
+    __shared__ double mm[R][kSaMomW];
+    if (threadIdx.x < nr) {
+        const int j = threadIdx.x;
         const int r = r0 + j;
         const float su = b.u[r];
         const double scale = LOG ? (double)exp2f(su) : (double)su;
@@ -275,15 +274,23 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         const double D[3] = {scale * tot[j * Q + 1], scale * tot[j * Q + 2],
                              scale * tot[j * Q + 3]};
         const double x[3] = {b.X[r], b.X[(size_t)N + r], b.X[2 * (size_t)N + r]};
-        m[0] += s;
+        mm[j][0] = s;
+#pragma unroll
         for (int i = 0; i < 3; ++i) {
-            m[1 + i] += s * x[i];
-            m[13 + i] += s * s * x[i];
-            m[16 + i] += D[i];
-            for (int jj = 0; jj < 3; ++jj) m[4 + 3 * i + jj] += s * x[i] * D[jj];
+            mm[j][1 + i] = s * x[i];
+            mm[j][13 + i] = s * s * x[i];
+            mm[j][16 + i] = D[i];
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) mm[j][4 + 3 * i + jj] = s * x[i] * D[jj];
         }
     }
-    for (int q = 0; q < kSaMomW; ++q) b.rmom[(size_t)q * gridDim.x + blockIdx.x] = m[q];
+    __syncthreads();
+    if (threadIdx.x < kSaMomW) {
+        const int q = threadIdx.x;
+        double acc = 0.0;
+        for (int j = 0; j < nr; ++j) acc += mm[j][q];
+        b.rmom[(size_t)q * gridDim.x + blockIdx.x] = acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
