@@ -55,18 +55,27 @@ struct SaPlan {
     int colBlocks = 0;  // ldk / 256
     int H = 0;          // row chunks per column block
     int chunkRows = 0;  // rows per chunk (multiple of kSaWaves)
+    int pH = 0;         // chunks of the rowBlocks partial rows of the one-pass sweep
+    int pChunkRows = 0; // partial rows per chunk (multiple of kSaWaves)
 };
+
+// Splits n rows into chunks so that colBlocks x chunks is about 4 workgroups
+// per CU; chunk lengths are multiples of kSaWaves.
+void split_rows(int n, int colBlocks, int numCU, int& H, int& chunkRows) {
+    const int target = std::max(1, numCU) * 4;
+    int h = std::max(1, (target + colBlocks - 1) / colBlocks);
+    h = std::min(h, (n + kSaWaves - 1) / kSaWaves);
+    chunkRows = ((n + h - 1) / h + kSaWaves - 1) / kSaWaves * kSaWaves;
+    H = (n + chunkRows - 1) / chunkRows;
+}
 
 SaPlan plan_sa(int N, int M, int numCU) {
     SaPlan p;
     p.ldk = (M + kSaColsPB - 1) / kSaColsPB * kSaColsPB;
     p.rowBlocks = (N + kSaRowsWG - 1) / kSaRowsWG;
     p.colBlocks = p.ldk / kSaColsPB;
-    const int target = std::max(1, numCU) * 4;
-    int H = std::max(1, (target + p.colBlocks - 1) / p.colBlocks);
-    H = std::min(H, (N + kSaWaves - 1) / kSaWaves);
-    p.chunkRows = ((N + H - 1) / H + kSaWaves - 1) / kSaWaves * kSaWaves;
-    p.H = (N + p.chunkRows - 1) / p.chunkRows;
+    split_rows(N, p.colBlocks, numCU, p.H, p.chunkRows);
+    split_rows(p.rowBlocks, p.colBlocks, numCU, p.pH, p.pChunkRows);
     return p;
 }
 
@@ -82,7 +91,7 @@ struct SaBufs {
     float* ocol;      // [M]
     float* bcol;      // [ldk] last column sums b_c
     float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
-    float* cpart;     // [H][ldk] column partial sums
+    float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [rowBlocks + pH][ldk])
     double* rmom;     // [kSaMomPad][rowBlocks] per-workgroup moments of the final pass
     double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
@@ -294,6 +303,150 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
 }
 
 // ---------------------------------------------------------------------------
+// Materialised variant, one Sinkhorn sweep in one pass over K: a workgroup
+// owns kSaRowsWG whole rows and keeps their K values in registers (S slabs of
+// 1024 columns: lane 4 columns per slab and row, ldk <= S * 1024). The row
+// sums with the current column scale give the new row scales (as the row
+// pass does), then the column partial sums over the workgroup's rows with the
+// new scales go to cpart[blockIdx.x] (as the column pass does over its row
+// chunk); sa_partsum_kernel reduces these rowBlocks partial rows to pH
+// chunk sums, which the column finalize adds up. K is read once per sweep
+// instead of twice. FIRST: the first sweep of an inner
+// iteration, which forms K from the transformed sources, stores it, and
+// starts from a fresh column scale v = 1.
+// ---------------------------------------------------------------------------
+constexpr int kSaFuseMaxSlabs = 6;
+
+template <bool FIRST, int S>
+__global__ void __launch_bounds__(kWave * kSaWaves)
+sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
+    constexpr int R = kSaRowsWG;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int r0 = blockIdx.x * R;
+    const int nr = min(R, N - r0);  // rows of this workgroup (uniform)
+
+    float xr[R][3];
+    double Rm[9], tm[3];
+    if (FIRST) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int r = r0 + min(j, nr - 1);
+            xr[j][0] = b.X[r];
+            xr[j][1] = b.X[(size_t)N + r];
+            xr[j][2] = b.X[2 * (size_t)N + r];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Rm[i] = b.state[kStR + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tm[i] = b.state[kStT + i];
+    }
+    float kv[S][R][4];
+    float acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc[j] = 0.f;
+#pragma unroll
+    for (int si = 0; si < S; ++si) {
+        const int c = 4 * lane + 4 * kWave * (wave + kSaWaves * si);
+        if (c >= ldk) {  // wave-uniform (ldk is a multiple of 256)
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) kv[si][j][i] = 0.f;
+            continue;
+        }
+        float vv[4] = {1.f, 1.f, 1.f, 1.f};
+        float p[3][4];
+        if (FIRST) {
+            float y[3][4];
+            load4(b.Yp + c, y[0]);
+            load4(b.Yp + (size_t)ldk + c, y[1]);
+            load4(b.Yp + 2 * (size_t)ldk + c, y[2]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double y0 = y[0][i], y1 = y[1][i], y2 = y[2][i];
+                const bool in = c + i < M;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    p[a][i] = in ? (float)(Rm[3 * a] * y0 + Rm[3 * a + 1] * y1 + Rm[3 * a + 2] * y2 +
+                                           tm[a])
+                                 : kSaSentinel;
+            }
+        } else {
+            load4(b.v + c, vv);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            float val[4] = {0.f, 0.f, 0.f, 0.f};
+            if (j < nr) {  // uniform
+                const int r = r0 + j;
+                if (FIRST) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float dx = xr[j][0] - p[0][i];
+                        const float dy = xr[j][1] - p[1][i];
+                        const float dz = xr[j][2] - p[2][i];
+                        float sq = dx * dx;
+                        sq = fmaf(dy, dy, sq);
+                        sq = fmaf(dz, dz, sq);
+                        val[i] = __builtin_amdgcn_exp2f(fmaf(sq, -k1, k0));
+                    }
+                    *reinterpret_cast<float4*>(b.K + (size_t)r * ldk + c) =
+                        make_float4(val[0], val[1], val[2], val[3]);
+                } else {
+                    load4(b.K + (size_t)r * ldk + c, val);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                kv[si][j][i] = val[i];
+                acc[j] = fmaf(val[i], vv[i], acc[j]);
+            }
+        }
+    }
+    // row sums: lane 63 of each wave, then the 4 waves in a fixed order;
+    // a_r = (row scale) * sum + o_row[r]; scale /= a_r; o_row /= a_r
+    __shared__ float red[kSaWaves][R];
+    __shared__ float unew[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const float t = wave_sum_last(acc[j]);
+        if (lane == kWave - 1) red[wave][j] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < nr) {
+        const int j = threadIdx.x;
+        const int r = r0 + j;
+        float tot = red[0][j];
+#pragma unroll
+        for (int w2 = 1; w2 < kSaWaves; ++w2) tot += red[w2][j];
+        const float su = FIRST ? 1.f : b.u[r];
+        const float a = su * tot + b.orow[r];
+        b.orow[r] = b.orow[r] / a;
+        const float un = su / a;
+        b.u[r] = un;
+        unew[j] = un;
+    }
+    __syncthreads();
+    float uu[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) uu[j] = (j < nr) ? unew[j] : 0.f;
+    // column partial sums over this workgroup's rows with the new row scales
+    float* cp = b.cpart + (size_t)blockIdx.x * ldk;
+#pragma unroll
+    for (int si = 0; si < S; ++si) {
+        const int c = 4 * lane + 4 * kWave * (wave + kSaWaves * si);
+        if (c >= ldk) continue;
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[i] = fmaf(kv[si][j][i], uu[j], part[i]);
+        *reinterpret_cast<float4*>(cp + c) = make_float4(part[0], part[1], part[2], part[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Column pass, stage 1: a workgroup = 256 columns x one row chunk; lanes own
 // 4 consecutive columns, the 4 waves split the chunk's rows.
 // ---------------------------------------------------------------------------
@@ -345,6 +498,39 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
 #pragma unroll
     for (int w = 1; w < kSaWaves; ++w) s += red[w][tid];
     b.cpart[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
+}
+
+// One-pass sweep, column sums stage 2: the rowBlocks partial rows of the
+// fused pass (in) reduced to pH partial rows (out) as stage 1 reduces K rows:
+// a workgroup = 256 columns x one chunk of partial rows, the 4 waves split the
+// chunk and combine in LDS in a fixed order.
+__global__ void __launch_bounds__(kWave * kSaWaves)
+sa_partsum_kernel(const float* __restrict__ in, float* __restrict__ out, int P, int ldk, int H,
+                  int chunkRows) {
+    const int cbk = blockIdx.x / H;
+    const int h = blockIdx.x - cbk * H;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int c = cbk * kSaColsPB + 4 * lane;
+    const int q = chunkRows / kSaWaves;
+    const int rs = h * chunkRows + wave * q;
+    const int re = min(rs + q, P);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int r = rs; r < re; ++r) {
+        float t[4];
+        load4(in + (size_t)r * ldk + c, t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] += t[i];
+    }
+    __shared__ float red[kSaWaves][kSaColsPB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][4 * lane + i] = acc[i];
+    __syncthreads();
+    const int tid = threadIdx.x;
+    float s = red[0][tid];
+#pragma unroll
+    for (int w = 1; w < kSaWaves; ++w) s += red[w][tid];
+    out[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
 }
 
 // Column pass, stage 2: a workgroup finalizes 64 columns; its 4 waves sum
@@ -478,6 +664,32 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     }
 }
 
+int launch_rowcol(bool first, int slabs, dim3 grid, dim3 blk, hipStream_t s, const SaBufs& b,
+                  int N, int M, int ldk, float k1, float k0) {
+#define EMSA_ROWCOL(S_)                                                                    \
+    case S_:                                                                               \
+        if (first)                                                                         \
+            hipLaunchKernelGGL((sa_rowcol_kernel<true, S_>), grid, blk, 0, s, b, N, M, ldk, \
+                               k1, k0);                                                    \
+        else                                                                               \
+            hipLaunchKernelGGL((sa_rowcol_kernel<false, S_>), grid, blk, 0, s, b, N, M,     \
+                               ldk, k1, k0);                                               \
+        break;
+    switch (slabs) {
+        EMSA_ROWCOL(1)
+        EMSA_ROWCOL(2)
+        EMSA_ROWCOL(3)
+        EMSA_ROWCOL(4)
+        EMSA_ROWCOL(5)
+        EMSA_ROWCOL(6)
+        default:
+            return set_error("softassign: bad slab count"), EMSA_EINVAL;
+    }
+#undef EMSA_ROWCOL
+    EMSA_HIP_TRY(hipGetLastError());
+    return EMSA_OK;
+}
+
 }  // namespace
 
 int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
@@ -508,7 +720,12 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.orow.reserve(sizeof(float) * ((size_t)N + M))) != EMSA_OK) return rc;
     b.orow = ctx.orow.as<float>();
     b.ocol = b.orow + N;
-    if ((rc = ctx.colpart.reserve(sizeof(float) * (size_t)pl.H * pl.ldk)) != EMSA_OK) return rc;
+    // materialised sweeps in one pass over K when a lane's share of its rows
+    // fits in registers (partials per row-pass workgroup); otherwise a row
+    // pass and a column pass (partials per row chunk)
+    const int slabs = (!LOGV && pl.ldk <= kSaFuseMaxSlabs * 1024) ? (pl.ldk + 1023) / 1024 : 0;
+    const size_t partRows = slabs > 0 ? (size_t)pl.rowBlocks + pl.pH : (size_t)pl.H;
+    if ((rc = ctx.colpart.reserve(sizeof(float) * partRows * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
                                 ((size_t)nmom * kSaMomPad + (size_t)finBlocks * 3))) != EMSA_OK)
@@ -518,12 +735,17 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
+    // one-pass sweep: the fused pass writes rowBlocks partial rows, stage 2
+    // reduces them to pH rows past them, which the finalize reads (bf: b with
+    // every field set, cpart moved)
+    SaBufs bf = b;
+    bf.cpart = b.cpart + (size_t)pl.rowBlocks * pl.ldk;
 
     hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
                        p.moutlier);
     EMSA_HIP_TRY(hipGetLastError());
     const dim3 rgrid((unsigned)pl.rowBlocks), cgrid((unsigned)(pl.colBlocks * pl.H)),
-        fgrid((unsigned)finBlocks);
+        fgrid((unsigned)finBlocks), pgrid((unsigned)(pl.colBlocks * pl.pH));
     const dim3 blk(kWave * kSaWaves);
     float T = p.T_0;
     int it = 0;
@@ -549,6 +771,18 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                     else
                         hipLaunchKernelGGL((sa_colfin_kernel<true, false>), fgrid, blk, 0, s, b,
                                            M, pl.ldk, pl.H, last);
+                } else if (slabs > 0) {
+                    if ((rc = launch_rowcol(sw == 0, slabs, rgrid, blk, s, b, N, M, pl.ldk, k1,
+                                            k0)) != EMSA_OK)
+                        return rc;
+                    hipLaunchKernelGGL(sa_partsum_kernel, pgrid, blk, 0, s, b.cpart, bf.cpart,
+                                       pl.rowBlocks, pl.ldk, pl.pH, pl.pChunkRows);
+                    if (sw == 0)
+                        hipLaunchKernelGGL((sa_colfin_kernel<false, true>), fgrid, blk, 0, s, bf,
+                                           M, pl.ldk, pl.pH, last);
+                    else
+                        hipLaunchKernelGGL((sa_colfin_kernel<false, false>), fgrid, blk, 0, s, bf,
+                                           M, pl.ldk, pl.pH, last);
                 } else {
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_rows_kernel<false, 0>), rgrid, blk, 0, s, b, N, M,
