@@ -516,6 +516,7 @@ sa_partsum_kernel(const float* __restrict__ in, float* __restrict__ out, int P, 
     const int rs = h * chunkRows + wave * q;
     const int re = min(rs + q, P);
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
     for (int r = rs; r < re; ++r) {
         float t[4];
         load4(in + (size_t)r * ldk + c, t);
@@ -546,8 +547,15 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int cc = blockIdx.x * kSaFinCols + lane;
-    double part = 0.0;
-    for (int k = wave; k < H; k += kSaWaves) part += b.cpart[(size_t)k * ldk + cc];
+    // four partial sums over interleaved chunks (four loads in flight per
+    // lane), added in a fixed order
+    double pa[4] = {0.0, 0.0, 0.0, 0.0};
+    int k = wave;
+    for (; k + 3 * kSaWaves < H; k += 4 * kSaWaves)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pa[u] += b.cpart[(size_t)(k + u * kSaWaves) * ldk + cc];
+    for (; k < H; k += kSaWaves) pa[0] += b.cpart[(size_t)k * ldk + cc];
+    const double part = (pa[0] + pa[1]) + (pa[2] + pa[3]);
     __shared__ double red[kSaWaves][kSaFinCols];
     red[wave][lane] = part;
     __syncthreads();
@@ -601,26 +609,41 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     const int lane = tid & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     // moments q = wave + 4 k (rows up to kSaMomPad - 1 exist, the ones past
-    // kSaMomW are not used): loads issued together, no per-quantity branch
+    // kSaMomW are not used): loads issued together, no per-quantity branch;
+    // kU partial sums per quantity so kU strides of partials are in flight,
+    // added in a fixed order at the end
     constexpr int kPer = kSaMomPad / kSaWaves;
-    double acc[kPer];
+    constexpr int kU = 4;
+    static_assert(kU == 4, "the final combine adds four partial sums");
+    double acc[kPer][kU];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
-    for (int w = lane; w < nmom; w += kWave)
+    for (int k = 0; k < kPer; ++k)
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) acc[k] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w];
+        for (int u = 0; u < kU; ++u) acc[k][u] = 0.0;
+    int w = lane;
+    for (; w + (kU - 1) * kWave < nmom; w += kU * kWave)
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+                acc[k][u] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w + u * kWave];
+    for (; w < nmom; w += kWave)
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) acc[k][0] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) acc[k][0] = (acc[k][0] + acc[k][1]) + (acc[k][2] + acc[k][3]);
     double by = 0.0;  // waves 0..2: component wave of sum b y
     if (wave < 3)
         for (int w = lane; w < nfin; w += kWave) by += b.cby[(size_t)wave * nfin + w];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) acc[k] = wave_sum(acc[k]);
+    for (int k = 0; k < kPer; ++k) acc[k][0] = wave_sum(acc[k][0]);
     by = wave_sum(by);
     __shared__ double mt[kSaMomW + 3];
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int q = wave + kSaWaves * k;
-            if (q < kSaMomW) mt[q] = acc[k];
+            if (q < kSaMomW) mt[q] = acc[k][0];
         }
         if (wave < 3) mt[kSaMomW + wave] = by;
     }
