@@ -44,6 +44,9 @@ namespace {
 
 constexpr int kSaWaves = 4;                  // waves per workgroup
 constexpr int kSaRowsWG = 4;                 // rows per row-pass workgroup (4 measured faster than 8, 16)
+constexpr int kSaFuseRows = 8;               // rows per workgroup of the one-pass materialised sweep
+constexpr int kSaFuseWaves = 8;              // its waves (they split the columns)
+constexpr int kSaFuseMaxSlabs = 3;           // column slabs of 256 kSaFuseWaves held in registers
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
 constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
 constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
@@ -55,7 +58,8 @@ struct SaPlan {
     int colBlocks = 0;  // ldk / 256
     int H = 0;          // row chunks per column block
     int chunkRows = 0;  // rows per chunk (multiple of kSaWaves)
-    int pH = 0;         // chunks of the rowBlocks partial rows of the one-pass sweep
+    int fuseBlocks = 0; // ceil(N / kSaFuseRows) workgroups of the one-pass sweep
+    int pH = 0;         // chunks of the fuseBlocks partial rows of the one-pass sweep
     int pChunkRows = 0; // partial rows per chunk (multiple of kSaWaves)
 };
 
@@ -75,7 +79,8 @@ SaPlan plan_sa(int N, int M, int numCU) {
     p.rowBlocks = (N + kSaRowsWG - 1) / kSaRowsWG;
     p.colBlocks = p.ldk / kSaColsPB;
     split_rows(N, p.colBlocks, numCU, p.H, p.chunkRows);
-    split_rows(p.rowBlocks, p.colBlocks, numCU, p.pH, p.pChunkRows);
+    p.fuseBlocks = (N + kSaFuseRows - 1) / kSaFuseRows;
+    split_rows(p.fuseBlocks, p.colBlocks, numCU, p.pH, p.pChunkRows);
     return p;
 }
 
@@ -91,7 +96,7 @@ struct SaBufs {
     float* ocol;      // [M]
     float* bcol;      // [ldk] last column sums b_c
     float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
-    float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [rowBlocks + pH][ldk])
+    float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [fuseBlocks + pH][ldk])
     double* rmom;     // [kSaMomPad][rowBlocks] per-workgroup moments of the final pass
     double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
@@ -304,23 +309,23 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
 
 // ---------------------------------------------------------------------------
 // Materialised variant, one Sinkhorn sweep in one pass over K: a workgroup
-// owns kSaRowsWG whole rows and keeps their K values in registers (S slabs of
-// 1024 columns: lane 4 columns per slab and row, ldk <= S * 1024). The row
+// of kSaFuseWaves waves owns kSaFuseRows whole rows and keeps their K values
+// in registers (S slabs of 256 kSaFuseWaves columns: lane 4 columns per slab
+// and row, ldk <= S * 256 kSaFuseWaves). The row
 // sums with the current column scale give the new row scales (as the row
 // pass does), then the column partial sums over the workgroup's rows with the
 // new scales go to cpart[blockIdx.x] (as the column pass does over its row
-// chunk); sa_partsum_kernel reduces these rowBlocks partial rows to pH
+// chunk); sa_partsum_kernel reduces these fuseBlocks partial rows to pH
 // chunk sums, which the column finalize adds up. K is read once per sweep
 // instead of twice. FIRST: the first sweep of an inner
 // iteration, which forms K from the transformed sources, stores it, and
 // starts from a fresh column scale v = 1.
 // ---------------------------------------------------------------------------
-constexpr int kSaFuseMaxSlabs = 6;
-
 template <bool FIRST, int S>
-__global__ void __launch_bounds__(kWave * kSaWaves)
+__global__ void __launch_bounds__(kWave * kSaFuseWaves) __attribute__((amdgpu_waves_per_eu(4)))
 sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
-    constexpr int R = kSaRowsWG;
+    constexpr int R = kSaFuseRows;
+    constexpr int NW = kSaFuseWaves;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int r0 = blockIdx.x * R;
@@ -347,7 +352,7 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     for (int j = 0; j < R; ++j) acc[j] = 0.f;
 #pragma unroll
     for (int si = 0; si < S; ++si) {
-        const int c = 4 * lane + 4 * kWave * (wave + kSaWaves * si);
+        const int c = 4 * lane + 4 * kWave * (wave + NW * si);
         if (c >= ldk) {  // wave-uniform (ldk is a multiple of 256)
 #pragma unroll
             for (int j = 0; j < R; ++j)
@@ -404,9 +409,9 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             }
         }
     }
-    // row sums: lane 63 of each wave, then the 4 waves in a fixed order;
+    // row sums: lane 63 of each wave, then the waves in a fixed order;
     // a_r = (row scale) * sum + o_row[r]; scale /= a_r; o_row /= a_r
-    __shared__ float red[kSaWaves][R];
+    __shared__ float red[NW][R];
     __shared__ float unew[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -419,7 +424,7 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         const int r = r0 + j;
         float tot = red[0][j];
 #pragma unroll
-        for (int w2 = 1; w2 < kSaWaves; ++w2) tot += red[w2][j];
+        for (int w2 = 1; w2 < NW; ++w2) tot += red[w2][j];
         const float su = FIRST ? 1.f : b.u[r];
         const float a = su * tot + b.orow[r];
         b.orow[r] = b.orow[r] / a;
@@ -435,7 +440,7 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     float* cp = b.cpart + (size_t)blockIdx.x * ldk;
 #pragma unroll
     for (int si = 0; si < S; ++si) {
-        const int c = 4 * lane + 4 * kWave * (wave + kSaWaves * si);
+        const int c = 4 * lane + 4 * kWave * (wave + NW * si);
         if (c >= ldk) continue;
         float part[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -500,7 +505,7 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
     b.cpart[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
 }
 
-// One-pass sweep, column sums stage 2: the rowBlocks partial rows of the
+// One-pass sweep, column sums stage 2: the fuseBlocks partial rows of the
 // fused pass (in) reduced to pH partial rows (out) as stage 1 reduces K rows:
 // a workgroup = 256 columns x one chunk of partial rows, the 4 waves split the
 // chunk and combine in LDS in a fixed order.
@@ -599,19 +604,27 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
         b.cby[(size_t)lane * gridDim.x + blockIdx.x] = (lane == 0) ? by[0] : (lane == 1 ? by[1] : by[2]);
 }
 
-// One workgroup: totals of the per-workgroup moments and of sum b y (fixed
-// order: wave w sums quantities w, w + 4, ..., its lanes striding over the
-// partials, then a wave sum), S, Horn and the new (R, t) in the state. The
-// next inner iteration's first row pass transforms the sources with it.
-__global__ void __launch_bounds__(kWave * kSaWaves)
+// One workgroup: totals of the per-workgroup moments and of sum b y, S, Horn
+// and the new (R, t) in the state. The next inner iteration's first row pass
+// transforms the sources with it. Wave w sums quantities q = w % 4 + 4 k over
+// the (w / 4)-th of kSaSolveWaves / 4 ranges of the partials (lanes striding,
+// four partial sums per lane so the loads are in flight together), then a
+// wave sum; the ranges are added in a fixed order in LDS.
+constexpr int kSaSolveWaves = 16;
+
+__global__ void __launch_bounds__(kWave * kSaSolveWaves)
 sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
+    constexpr int kParts = kSaSolveWaves / kSaWaves;
+    static_assert(kParts * kSaWaves == kSaSolveWaves, "whole quantity classes per range");
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    // moments q = wave + 4 k (rows up to kSaMomPad - 1 exist, the ones past
-    // kSaMomW are not used): loads issued together, no per-quantity branch;
-    // kU partial sums per quantity so kU strides of partials are in flight,
-    // added in a fixed order at the end
+    const int qc = wave % kSaWaves;
+    const int part = wave / kSaWaves;
+    const int len = (nmom + kParts - 1) / kParts;
+    const int wbeg = part * len, wend = min(wbeg + len, nmom);
+    // rows up to kSaMomPad - 1 exist (the ones past kSaMomW are not used):
+    // no per-quantity branch
     constexpr int kPer = kSaMomPad / kSaWaves;
     constexpr int kU = 4;
     static_assert(kU == 4, "the final combine adds four partial sums");
@@ -620,32 +633,41 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     for (int k = 0; k < kPer; ++k)
 #pragma unroll
         for (int u = 0; u < kU; ++u) acc[k][u] = 0.0;
-    int w = lane;
-    for (; w + (kU - 1) * kWave < nmom; w += kU * kWave)
+    int w = wbeg + lane;
+    for (; w + (kU - 1) * kWave < wend; w += kU * kWave)
 #pragma unroll
         for (int u = 0; u < kU; ++u)
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
-                acc[k][u] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w + u * kWave];
-    for (; w < nmom; w += kWave)
+                acc[k][u] += b.rmom[(size_t)(qc + kSaWaves * k) * nmom + w + u * kWave];
+    for (; w < wend; w += kWave)
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) acc[k][0] += b.rmom[(size_t)(wave + kSaWaves * k) * nmom + w];
+        for (int k = 0; k < kPer; ++k) acc[k][0] += b.rmom[(size_t)(qc + kSaWaves * k) * nmom + w];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) acc[k][0] = (acc[k][0] + acc[k][1]) + (acc[k][2] + acc[k][3]);
-    double by = 0.0;  // waves 0..2: component wave of sum b y
-    if (wave < 3)
-        for (int w = lane; w < nfin; w += kWave) by += b.cby[(size_t)wave * nfin + w];
+    const bool byWave = (part == 0 && qc < 3);  // component qc of sum b y
+    double by = 0.0;
+    if (byWave)
+        for (int c = lane; c < nfin; c += kWave) by += b.cby[(size_t)qc * nfin + c];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) acc[k][0] = wave_sum(acc[k][0]);
     by = wave_sum(by);
+    __shared__ double red[kParts][kSaMomPad];
+    __shared__ double rby[3];
     __shared__ double mt[kSaMomW + 3];
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int q = wave + kSaWaves * k;
-            if (q < kSaMomW) mt[q] = acc[k][0];
-        }
-        if (wave < 3) mt[kSaMomW + wave] = by;
+        for (int k = 0; k < kPer; ++k) red[part][qc + kSaWaves * k] = acc[k][0];
+        if (byWave) rby[qc] = by;
+    }
+    __syncthreads();
+    if (tid < kSaMomW) {
+        double t = red[0][tid];
+#pragma unroll
+        for (int pp = 1; pp < kParts; ++pp) t += red[pp][tid];
+        mt[tid] = t;
+    } else if (tid < kSaMomW + 3) {
+        mt[tid] = rby[tid - kSaMomW];
     }
     __syncthreads();
     if (tid == 0) {
@@ -687,28 +709,25 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     }
 }
 
+template <int S>
+bool launch_rowcol_upto(int slabs, bool first, dim3 grid, dim3 blk, hipStream_t s,
+                        const SaBufs& b, int N, int M, int ldk, float k1, float k0) {
+    if (slabs == S) {
+        if (first)
+            hipLaunchKernelGGL((sa_rowcol_kernel<true, S>), grid, blk, 0, s, b, N, M, ldk, k1, k0);
+        else
+            hipLaunchKernelGGL((sa_rowcol_kernel<false, S>), grid, blk, 0, s, b, N, M, ldk, k1, k0);
+        return true;
+    }
+    if constexpr (S < kSaFuseMaxSlabs)
+        return launch_rowcol_upto<S + 1>(slabs, first, grid, blk, s, b, N, M, ldk, k1, k0);
+    return false;
+}
+
 int launch_rowcol(bool first, int slabs, dim3 grid, dim3 blk, hipStream_t s, const SaBufs& b,
                   int N, int M, int ldk, float k1, float k0) {
-#define EMSA_ROWCOL(S_)                                                                    \
-    case S_:                                                                               \
-        if (first)                                                                         \
-            hipLaunchKernelGGL((sa_rowcol_kernel<true, S_>), grid, blk, 0, s, b, N, M, ldk, \
-                               k1, k0);                                                    \
-        else                                                                               \
-            hipLaunchKernelGGL((sa_rowcol_kernel<false, S_>), grid, blk, 0, s, b, N, M,     \
-                               ldk, k1, k0);                                               \
-        break;
-    switch (slabs) {
-        EMSA_ROWCOL(1)
-        EMSA_ROWCOL(2)
-        EMSA_ROWCOL(3)
-        EMSA_ROWCOL(4)
-        EMSA_ROWCOL(5)
-        EMSA_ROWCOL(6)
-        default:
-            return set_error("softassign: bad slab count"), EMSA_EINVAL;
-    }
-#undef EMSA_ROWCOL
+    if (!launch_rowcol_upto<1>(slabs, first, grid, blk, s, b, N, M, ldk, k1, k0))
+        return set_error("softassign: bad slab count"), EMSA_EINVAL;
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
 }
@@ -746,8 +765,10 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     // materialised sweeps in one pass over K when a lane's share of its rows
     // fits in registers (partials per row-pass workgroup); otherwise a row
     // pass and a column pass (partials per row chunk)
-    const int slabs = (!LOGV && pl.ldk <= kSaFuseMaxSlabs * 1024) ? (pl.ldk + 1023) / 1024 : 0;
-    const size_t partRows = slabs > 0 ? (size_t)pl.rowBlocks + pl.pH : (size_t)pl.H;
+    constexpr int slabCols = kSaColsPB * kSaFuseWaves;
+    const int slabs =
+        (!LOGV && pl.ldk <= kSaFuseMaxSlabs * slabCols) ? (pl.ldk + slabCols - 1) / slabCols : 0;
+    const size_t partRows = slabs > 0 ? (size_t)pl.fuseBlocks + pl.pH : (size_t)pl.H;
     if ((rc = ctx.colpart.reserve(sizeof(float) * partRows * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
@@ -758,11 +779,11 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
-    // one-pass sweep: the fused pass writes rowBlocks partial rows, stage 2
+    // one-pass sweep: the fused pass writes fuseBlocks partial rows, stage 2
     // reduces them to pH rows past them, which the finalize reads (bf: b with
     // every field set, cpart moved)
     SaBufs bf = b;
-    bf.cpart = b.cpart + (size_t)pl.rowBlocks * pl.ldk;
+    bf.cpart = b.cpart + (size_t)pl.fuseBlocks * pl.ldk;
 
     hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
                        p.moutlier);
@@ -770,6 +791,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     const dim3 rgrid((unsigned)pl.rowBlocks), cgrid((unsigned)(pl.colBlocks * pl.H)),
         fgrid((unsigned)finBlocks), pgrid((unsigned)(pl.colBlocks * pl.pH));
     const dim3 blk(kWave * kSaWaves);
+    const dim3 xgrid((unsigned)pl.fuseBlocks), xblk(kWave * kSaFuseWaves);
     float T = p.T_0;
     int it = 0;
     for (int Titer = 1; Titer <= p.JMAX; ++Titer) {
@@ -795,11 +817,11 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                         hipLaunchKernelGGL((sa_colfin_kernel<true, false>), fgrid, blk, 0, s, b,
                                            M, pl.ldk, pl.H, last);
                 } else if (slabs > 0) {
-                    if ((rc = launch_rowcol(sw == 0, slabs, rgrid, blk, s, b, N, M, pl.ldk, k1,
+                    if ((rc = launch_rowcol(sw == 0, slabs, xgrid, xblk, s, b, N, M, pl.ldk, k1,
                                             k0)) != EMSA_OK)
                         return rc;
                     hipLaunchKernelGGL(sa_partsum_kernel, pgrid, blk, 0, s, b.cpart, bf.cpart,
-                                       pl.rowBlocks, pl.ldk, pl.pH, pl.pChunkRows);
+                                       pl.fuseBlocks, pl.ldk, pl.pH, pl.pChunkRows);
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_colfin_kernel<false, true>), fgrid, blk, 0, s, bf,
                                            M, pl.ldk, pl.pH, last);
@@ -830,7 +852,8 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
                                    k1, k0);
-            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), blk, 0, s, b, nmom, finBlocks, T, it);
+            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), dim3(kWave * kSaSolveWaves), 0, s, b,
+                               nmom, finBlocks, T, it);
             EMSA_HIP_TRY(hipGetLastError());
         }
         T = T * p.TFACTOR;  // softassign.cu:820, float
