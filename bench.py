@@ -42,9 +42,9 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
 # (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
 # FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
-# WRITE_SIZE, run on `bench.py --counters affinity`; 14.9 MiB fetched (each of
-# the 8 L2s reads the ~1.9 MB of inputs) + 7.9 MiB of partial sums written
-AFFINITY_TRAFFIC_BYTES = 23.9e6
+# WRITE_SIZE, run on `bench.py --counters affinity`; 15.6 MB fetched (each of
+# the 8 L2s reads the ~1.9 MB of inputs) + 8.3 MB of partial sums written
+AFFINITY_TRAFFIC_BYTES = 24.0e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
@@ -242,8 +242,10 @@ def softassign_plain(E, n=5032):
     """Softassign at the reference defaults (JMAX 100, I0 5, I1 3, alpha 3,
     T_0 100, TFACTOR 0.95: 500 inner iterations), N = M = 5032 surface pairs
     with 10% outliers, both variants, one registration each after a warm-up.
-    Algorithmic work per inner iteration: materialised 4 (1 + 2 I1) = 28 bytes
-    per matrix element; log-domain 2 I1 + 1 = 7 exp per element."""
+    Algorithmic work per inner iteration: materialised 4 (I1 + 1) = 16 bytes
+    per matrix element (K written once by the first sweep, read once by each
+    later sweep and by the final row pass); log-domain 2 I1 + 1 = 7 exp per
+    element."""
     from cuda_emicp_softassign_amd.clouds import surface_with_outliers
     X, Y, _, _ = surface_with_outliers(n, seed=1)
     p = E.registrationParameters()
@@ -258,7 +260,7 @@ def softassign_plain(E, n=5032):
         ms = (time.perf_counter() - t0) * 1e3
         r = {"ms_per_registration": ms, "inner_iterations_per_s": inner * 1e3 / ms}
         if variant == "materialised":
-            r["algorithmic_GB_per_s"] = 4.0 * (1 + 2 * p.I1) * elems * inner / (ms * 1e-3) / 1e9
+            r["algorithmic_GB_per_s"] = 4.0 * (p.I1 + 1) * elems * inner / (ms * 1e-3) / 1e9
             r["frac_of_hbm_peak"] = r["algorithmic_GB_per_s"] / HBM_PEAK_GBS
         else:
             r["algorithmic_exp_per_s"] = (2 * p.I1 + 1) * elems * inner / (ms * 1e-3)
@@ -506,7 +508,7 @@ def main():
             "bound": "mfma",
             "compute": "FP32 VALU (difference, |d|^2, C) + v_exp_f32; B and W as a product "
                        "over the column dimension on the matrix cores (v_mfma_f32_4x4x1f32)",
-            "kernel": "emicp_affinity_kernel",
+            "kernel": "emicp_affinity_mfma_kernel",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
