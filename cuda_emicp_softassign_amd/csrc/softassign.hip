@@ -47,6 +47,8 @@ constexpr int kSaRowsWG = 4;                 // rows per row-pass workgroup (4 m
 constexpr int kSaFuseRows = 8;               // rows per workgroup of the one-pass materialised sweep
 constexpr int kSaFuseWaves = 8;              // its waves (they split the columns)
 constexpr int kSaFuseMaxSlabs = 3;           // column slabs of 256 kSaFuseWaves held in registers
+constexpr int kSaFinalRows = 8;              // materialised final pass: rows per workgroup
+constexpr int kSaFinalWaves = 8;             // and its waves
 constexpr int kSaColsPB = 256;               // columns per column-pass workgroup (64 lanes x 4)
 constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
 constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
@@ -97,7 +99,7 @@ struct SaBufs {
     float* bcol;      // [ldk] last column sums b_c
     float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
     float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [fuseBlocks + pH][ldk])
-    double* rmom;     // [kSaMomPad][rowBlocks] per-workgroup moments of the final pass
+    double* rmom;     // [kSaMomPad][final-pass workgroups] moments of the final pass
     double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
     double* trace;    // nullable
@@ -135,11 +137,10 @@ __device__ inline void load4(const float* p, float (&v)[4]) {
 // final pass (s_r, D'_r and the workgroup's FP64 moments).
 // ---------------------------------------------------------------------------
 
-template <bool LOG, int MODE>
-__global__ void __launch_bounds__(kWave * kSaWaves)
+template <bool LOG, int MODE, int R = kSaRowsWG, int NW = kSaWaves>
+__global__ void __launch_bounds__(kWave * NW)
 sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     constexpr int Q = (MODE == 2) ? 4 : 1;
-    constexpr int R = kSaRowsWG;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int r0 = blockIdx.x * R;
@@ -169,7 +170,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         for (int i = 0; i < 3; ++i) tm[i] = b.state[kStT + i];
     }
 
-    for (int c = 4 * lane + 4 * kWave * wave; c < ldk; c += 4 * kWave * kSaWaves) {
+    for (int c = 4 * lane + 4 * kWave * wave; c < ldk; c += 4 * kWave * NW) {
         float w[4][Q];  // per-column weights
         float p[3][4];
         float kg[4];
@@ -242,8 +243,8 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
                 for (int q = 0; q < Q; ++q) acc[j][q] = fmaf(val[i], w[i][q], acc[j][q]);
         }
     }
-    // row sums: lane 63 of each wave, then the 4 waves in a fixed order
-    __shared__ float red[kSaWaves][R * Q];
+    // row sums: lane 63 of each wave, then the waves in a fixed order
+    __shared__ float red[NW][R * Q];
 #pragma unroll
     for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -256,7 +257,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     if (threadIdx.x < R * Q) {
         float v = red[0][threadIdx.x];
 #pragma unroll
-        for (int w2 = 1; w2 < kSaWaves; ++w2) v += red[w2][threadIdx.x];
+        for (int w2 = 1; w2 < NW; ++w2) v += red[w2][threadIdx.x];
         tot[threadIdx.x] = v;
     }
     __syncthreads();
@@ -739,7 +740,9 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                       hipStream_t s, double* trace) {
     const bool LOGV = (variant == EMSA_SOFTASSIGN_LOGDOMAIN);
     const SaPlan pl = plan_sa(N, M, ctx.numCU);
-    const int nmom = pl.rowBlocks;            // moment rows of the final pass
+    // moment rows of the final pass (one per final-pass workgroup)
+    const int finalBlocks = LOGV ? pl.rowBlocks : (N + kSaFinalRows - 1) / kSaFinalRows;
+    const int nmom = finalBlocks;
     const int finBlocks = pl.ldk / kSaFinCols;  // column-finalize workgroups
     int rc;
     SaBufs b{};
@@ -850,8 +853,9 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                 hipLaunchKernelGGL((sa_rows_kernel<true, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
                                    k1, k0);
             else
-                hipLaunchKernelGGL((sa_rows_kernel<false, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
-                                   k1, k0);
+                hipLaunchKernelGGL((sa_rows_kernel<false, 2, kSaFinalRows, kSaFinalWaves>),
+                                   dim3((unsigned)finalBlocks), dim3(kWave * kSaFinalWaves), 0, s,
+                                   b, N, M, pl.ldk, k1, k0);
             hipLaunchKernelGGL(sa_solve_kernel, dim3(1), dim3(kWave * kSaSolveWaves), 0, s, b,
                                nmom, finBlocks, T, it);
             EMSA_HIP_TRY(hipGetLastError());

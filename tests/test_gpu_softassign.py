@@ -60,12 +60,16 @@ def test_default_parameters_match_oracle(E, oracle, variant):
 
 @pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("N,M", [(7, 4), (9, 12), (20, 300), (300, 20), (65, 64), (257, 513),
-                                 (100, 1000), (1000, 100)])
+                                 (100, 1000), (1000, 100), (300, 2049), (600, 6144),
+                                 (600, 6145)])
 def test_ragged_sizes(E, oracle, variant, N, M):
     """Uneven and tiny sizes; Y is a moved, noisy resampling of X so that
     Horn's problem is well posed (for unrelated clouds, or a handful of points
     matched by hundreds, the top eigenvalue of Horn's matrix is nearly
-    degenerate and the FP32 and FP64 oracles themselves part by radians)."""
+    degenerate and the FP32 and FP64 oracles themselves part by radians).
+    The materialised sweep holds up to 3 slabs of 2048 columns in registers:
+    M = 2049 leaves a partly empty slab, 6144 fills all three, 6145 takes the
+    separate row and column passes."""
     rng = np.random.default_rng(N + 31 * M)
     X = rng.uniform(-0.5, 0.5, size=(N, 3))
     Y = (X[rng.integers(0, N, size=M)] - 0.02) @ rotation([1, 0, 1], 0.15)
