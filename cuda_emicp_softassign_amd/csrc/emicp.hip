@@ -13,10 +13,11 @@
 // and A' = e / sqrt(C_r)) each lane (one source row) accumulates in registers
 //   C_r = sum e^2,  B_r = sum e,  W_r = sum e x_c
 // and then lambda_r = B_r / sqrt(C_r), lambda_r X'_r = W_r / sqrt(C_r).
-// Target points are wave-uniform, so they are read with scalar loads and
-// enter the VALU as SGPR operands; one exp2 and 15 VALU ops per pair, or 10
-// when B and W are summed on the matrix cores and the exponent scale is
-// carried by the coordinates (the default kernel).
+// Target points are wave-uniform: the all-VALU kernel reads them with scalar
+// loads (SGPR operands; one exp2 and 15 VALU ops per pair); the default
+// kernel reads them from its LDS stage with broadcast reads and needs 10 VALU
+// ops per pair (B and W summed on the matrix cores, the exponent scale carried
+// by the coordinates).
 //
 // Work decomposition per sigma^2 step, three launches with no signalling
 // between workgroups (kernel boundaries order them):
@@ -665,7 +666,7 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 
 template <bool COMP, int UNR, int NACC>
-__global__ void __launch_bounds__(kEmRows * kEmWaves)
+__global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
 emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
                            unsigned long long* __restrict__ visits) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
@@ -786,22 +787,31 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
             for (int v = 0; v < UNR / 4; ++v)
                 bq[v] = *reinterpret_cast<const f4*>(&st[j][c + 4 * v]);
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int cc = cc0 + u;
-                float dx = fmaf(Xx[cc], ka, -qx), dy = fmaf(Xy[cc], ka, -qy),
-                      dz = fmaf(Xz[cc], ka, -qz);
-                if (COMP) {
-                    dx -= qlx;
-                    dy -= qly;
-                    dz -= qlz;
+            for (int h = 0; h < UNR / 4; ++h) {
+                // the coordinates of 4 columns, read by every lane from the
+                // stage at one address (an LDS broadcast): no scalar-memory
+                // round trip per visited group
+                const f4 xs = *reinterpret_cast<const f4*>(&st[0][c + 4 * h]);
+                const f4 ys = *reinterpret_cast<const f4*>(&st[1][c + 4 * h]);
+                const f4 zs = *reinterpret_cast<const f4*>(&st[2][c + 4 * h]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int u = 4 * h + i;
+                    float dx = fmaf(xs[i], ka, -qx), dy = fmaf(ys[i], ka, -qy),
+                          dz = fmaf(zs[i], ka, -qz);
+                    if (COMP) {
+                        dx -= qlx;
+                        dy -= qly;
+                        dz -= qlz;
+                    }
+                    float s = dx * dx;
+                    s = fmaf(dy, dy, s);
+                    s = fmaf(dz, dz, s);
+                    const float e = __builtin_amdgcn_exp2f(-s);
+                    C = fmaf(e, e, C);
+                    D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[h][i], D[u % NACC],
+                                                                     0, 0, 0);
                 }
-                float s = dx * dx;
-                s = fmaf(dy, dy, s);
-                s = fmaf(dz, dz, s);
-                const float e = __builtin_amdgcn_exp2f(-s);
-                C = fmaf(e, e, C);
-                D[u % NACC] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, bq[u >> 2][u & 3],
-                                                                 D[u % NACC], 0, 0, 0);
             }
         }
         wave_lds_sync();
