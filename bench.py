@@ -360,8 +360,8 @@ def main():
 
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the sharding design is described with the kernels)", file=sys.stderr)
         sys.exit(2)
 
     import torch
