@@ -479,41 +479,23 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 
 // Stage 2, one wave and one row block (64 source rows) of registration d:
 // the wave-summed 18 FP64 moments of the block, valid in every lane.
-__device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, double outlier,
-                                         int last, int iter, double (&m)[kMomW]) {
-    const int N = d.N, M = d.M, chunks = d.chunks;
+// Row finalize of row rb * 64 + lane from its sums over all target columns
+// (C without the outlier term, B, W): the floor, lambda, lambda X', the
+// per-row outputs and checkpoint; then the 18 FP64 moments of the row block,
+// each summed over the wave as it is formed (one live at a time) and stored
+// by lane q (moment-major).
+//  [0] lambda  [1..3] lambda X'  [4..6] lambda y  [7 + 3i + j] lambda X'_i y_j
+//  [16] lambda |X' - (R y + t)|^2  [17] floored rows
+// SERIAL: one moment formed and summed at a time (few registers, for a
+// kernel held to 8 waves per SIMD); otherwise all 18 sums are interleaved.
+template <bool SERIAL>
+__device__ __attribute__((always_inline)) inline void row_block_finalize(const EmRegDesc& d, int rb, int lane, double Cs,
+                                          double Bs, double Xs, double Ys, double Zs,
+                                          double outlier, int last, int iter) {
+    const int N = d.N, M = d.M;
     const int r = rb * kEmRows + lane;
     const bool valid = r < M;
     const double* __restrict__ state = d.state;
-
-    // chunk partials of this row block, summed in chunk order in FP64; the
-    // loads of 4 chunks are issued before their adds
-    double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
-    const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
-    int k = 0;
-    for (; k + 4 <= chunks; k += 4) {
-        float v[4][5];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int q = 0; q < 5; ++q) v[i][q] = Pr[((size_t)(k + i) * 5 + q) * kEmRows];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            Cs += v[i][0];
-            Bs += v[i][1];
-            Xs += v[i][2];
-            Ys += v[i][3];
-            Zs += v[i][4];
-        }
-    }
-    for (; k < chunks; ++k) {
-        const float* q = Pr + (size_t)k * 5 * kEmRows;
-        Cs += q[0];
-        Bs += q[kEmRows];
-        Xs += q[2 * kEmRows];
-        Ys += q[3 * kEmRows];
-        Zs += q[4 * kEmRows];
-    }
     Cs += outlier;
     double lam, Vx, Vy, Vz, fl = 0.0;
     if ((float)Cs > 10e-7f) {  // emicp.cu:155, compared in float
@@ -553,28 +535,73 @@ __device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, d
     }
     double pxd, pyd, pzd;
     transform_row(state, yx, yy, yz, pxd, pyd, pzd);
-    m[0] = lam;
-    m[1] = Vx;
-    m[2] = Vy;
-    m[3] = Vz;
-    m[4] = lam * yx;
-    m[5] = lam * yy;
-    m[6] = lam * yz;
-    const double V[3] = {Vx, Vy, Vz};
-    const double yv[3] = {(double)yx, (double)yy, (double)yz};
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) m[7 + 3 * i + j] = V[i] * yv[j];
     double res = 0.0;
     if (lam > 0.0) {
         const double ex = Vx / lam - pxd, ey = Vy / lam - pyd, ez = Vz / lam - pzd;
         res = lam * (ex * ex + ey * ey + ez * ez);
     }
-    m[16] = res;
-    m[17] = fl;
+    const double y0 = yx, y1 = yy, y2 = yz;
+    double mine = 0.0;  // moment `lane` of the row block
+    auto emit = [&](int q, double v) {
+        v = wave_sum(v);
+        if (lane == q) mine = v;
+        // serial: keep the scheduler from interleaving the 18 sums
+        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
+    };
+    emit(0, lam);
+    emit(1, Vx);
+    emit(2, Vy);
+    emit(3, Vz);
+    emit(4, lam * y0);
+    emit(5, lam * y1);
+    emit(6, lam * y2);
+    emit(7, Vx * y0);
+    emit(8, Vx * y1);
+    emit(9, Vx * y2);
+    emit(10, Vy * y0);
+    emit(11, Vy * y1);
+    emit(12, Vy * y2);
+    emit(13, Vz * y0);
+    emit(14, Vz * y1);
+    emit(15, Vz * y2);
+    emit(16, res);
+    emit(17, fl);
+    static_assert(kMomW == 18, "moment layout above");
+    if (lane < kMomW) d.moments[(size_t)lane * d.rowBlocks + rb] = mine;
+}
+
+// The row block's sums from the chunk partials (in chunk order, FP64; the
+// loads of 4 chunks are issued before their adds), then the finalize.
+__device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, double outlier,
+                                         int last, int iter) {
+    const int chunks = d.chunks;
+    double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
+    const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
+    int k = 0;
+    for (; k + 4 <= chunks; k += 4) {
+        float v[4][5];
 #pragma unroll
-    for (int q = 0; q < kMomW; ++q) m[q] = wave_sum(m[q]);
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v[i][q] = Pr[((size_t)(k + i) * 5 + q) * kEmRows];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            Cs += v[i][0];
+            Bs += v[i][1];
+            Xs += v[i][2];
+            Ys += v[i][3];
+            Zs += v[i][4];
+        }
+    }
+    for (; k < chunks; ++k) {
+        const float* q = Pr + (size_t)k * 5 * kEmRows;
+        Cs += q[0];
+        Bs += q[kEmRows];
+        Xs += q[2 * kEmRows];
+        Ys += q[3 * kEmRows];
+        Zs += q[4 * kEmRows];
+    }
+    row_block_finalize<false>(d, rb, lane, Cs, Bs, Xs, Ys, Zs, outlier, last, iter);
 }
 
 // Stage 2: one wave per row block; moments to HBM (moment-major).
@@ -588,15 +615,7 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     const int ri = find_reg_rows(regs, nregs, (int)gr);
     const EmRegDesc& d = regs[ri];
     const int rb = (int)gr - d.rowBase;
-    double m[kMomW];
-    row_block_moments(d, rb, lane, outlier, last, iter, m);
-    if (lane < kMomW) {
-        double v = m[0];
-#pragma unroll
-        for (int q = 1; q < kMomW; ++q)
-            if (lane == q) v = m[q];
-        d.moments[(size_t)lane * d.rowBlocks + rb] = v;
-    }
+    row_block_moments(d, rb, lane, outlier, last, iter);
 }
 
 // Stage 3 for registration d from its 18 summed moments (one thread):
@@ -665,10 +684,11 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // number of (valid row, valid column) pairs the launch evaluates.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 
-template <bool COMP, int UNR, int NACC>
+template <bool COMP, int UNR, int NACC, bool FUSE>
 __global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
 emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
-                           unsigned long long* __restrict__ visits) {
+                           unsigned long long* __restrict__ visits, double outlier, int last,
+                           int iter) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -835,13 +855,23 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     for (int i = 0; i < 4; ++i) red[q * kEmRows + rb4 + i] = D[0][i];
     __syncthreads();
     if (wave == 0) {
-        float* P = d.partial + (size_t)local * 5 * kEmRows;
+        float acc[5];
 #pragma unroll
         for (int qq = 0; qq < 5; ++qq) {
-            float acc = stage[0][0][qq * kEmRows + lane];
+            acc[qq] = stage[0][0][qq * kEmRows + lane];
 #pragma unroll
-            for (int w = 1; w < kEmWaves; ++w) acc += stage[w][0][qq * kEmRows + lane];
-            P[qq * kEmRows + lane] = acc;
+            for (int w = 1; w < kEmWaves; ++w) acc[qq] += stage[w][0][qq * kEmRows + lane];
+        }
+        if (FUSE) {
+            // the tile spans the whole target (one chunk): these are the
+            // row sums the row finalize would read back, so it runs here
+            row_block_finalize<true>(d, rb, lane, (double)acc[0], (double)acc[1],
+                                     (double)acc[2], (double)acc[3], (double)acc[4], outlier,
+                                     last, iter);
+        } else {
+            float* P = d.partial + (size_t)local * 5 * kEmRows;
+#pragma unroll
+            for (int qq = 0; qq < 5; ++qq) P[qq * kEmRows + lane] = acc[qq];
         }
     }
 }
@@ -888,18 +918,26 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
 // kCompBelowSigma2; B and W on the matrix cores unless the context selects
 // the all-VALU kernel.
 void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
-                     float sigma_p2, hipStream_t s, unsigned long long* visits) {
+                     float sigma_p2, hipStream_t s, unsigned long long* visits,
+                     const EmRowFuse* fuse) {
     const dim3 blk(kEmRows * kEmWaves);
     const bool comp = sigma_p2 < kCompBelowSigma2;
     if (ctx.emAffinityMfma) {
         // exponent scale sqrt(log2 e / (2 sigma^2)), applied to the coordinates
         const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
-        if (comp)
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<true, 8, 4>), dim3(tiles), blk,
-                               0, s, desc, nregs, ka, ctx.emCull, visits);
-        else
-            hipLaunchKernelGGL((emicp_affinity_mfma_kernel<false, 8, 4>), dim3(tiles), blk,
-                               0, s, desc, nregs, ka, ctx.emCull, visits);
+        const double outlier = fuse ? fuse->outlier : 0.0;
+        const int last = fuse ? fuse->last : 0, iter = fuse ? fuse->iter : 0;
+#define EMSA_AFF(C_, F_)                                                                     \
+    hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 4, F_>), dim3(tiles), blk, 0, s, desc, \
+                       nregs, ka, ctx.emCull, visits, outlier, last, iter)
+        if (fuse) {
+            if (comp) EMSA_AFF(true, true);
+            else EMSA_AFF(false, true);
+        } else {
+            if (comp) EMSA_AFF(true, false);
+            else EMSA_AFF(false, false);
+        }
+#undef EMSA_AFF
     } else {
         const float negk = (float)(-kLog2e / (2.0 * (double)sigma_p2));
         if (comp)
@@ -1011,6 +1049,8 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     if (tiles > 0x7fffffffLL) return set_error("emicp: batch too large for one launch"), EMSA_EINVAL;
     w.tiles = tiles;
     w.rowBlocks = rowBlocks;
+    w.singleChunk = true;
+    for (const EmRegDesc& d : w.regs) w.singleChunk = w.singleChunk && d.chunks == 1;
     const int nregs = (int)w.regs.size();
     int rc;
     if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
@@ -1058,18 +1098,25 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     int rc = em_launch_setup(w, dRt0, s);
     if (rc != EMSA_OK) return rc;
     const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
+    // every tile spans its whole target: the affinity kernel finalizes the
+    // rows itself and the row kernel is not launched
+    const bool fuse = ctx.emRowFuse && ctx.emAffinityMfma && w.singleChunk;
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it], s));
+        const EmRowFuse fz{outlier, last, (int)it};
         launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig, s,
-                        (probe && probe->visits) ? probe->visits + it : nullptr);
+                        (probe && probe->visits) ? probe->visits + it : nullptr,
+                        fuse ? &fz : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
-        hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s, desc,
-                           nregs, w.rowBlocks, outlier, last, (int)it);
-        EMSA_HIP_TRY(hipGetLastError());
+        if (!fuse) {
+            hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s,
+                               desc, nregs, w.rowBlocks, outlier, last, (int)it);
+            EMSA_HIP_TRY(hipGetLastError());
+        }
         hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,
                            sig, (int)it);
         EMSA_HIP_TRY(hipGetLastError());
@@ -1086,7 +1133,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
 int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s,
                        unsigned long long* visits) {
     launch_affinity(ctx, w.desc.as<EmRegDesc>(), (int)w.regs.size(), (unsigned)w.tiles,
-                    sigma_p2, s, visits);
+                    sigma_p2, s, visits, nullptr);
     EMSA_HIP_TRY(hipGetLastError());
     return EMSA_OK;
 }

@@ -100,8 +100,17 @@ struct EmPair {
 struct EmWork {
     DevBuf xpad, partial, moments, state, desc, order;
     std::vector<EmRegDesc> regs;
-    long long tiles = 0;      // affinity workgroups per step
-    long long rowBlocks = 0;  // row blocks per step over the whole table
+    long long tiles = 0;       // affinity workgroups per step
+    long long rowBlocks = 0;   // row blocks per step over the whole table
+    bool singleChunk = false;  // every registration's tile spans its whole target
+};
+
+// What the affinity kernel needs to finalize the rows itself (one chunk per
+// row block): the step's outlier term, whether it is the last step, its index.
+struct EmRowFuse {
+    double outlier;
+    int last;
+    int iter;
 };
 
 struct GraphEntry {
@@ -142,6 +151,7 @@ struct DeviceCtx {
     int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
     int emAffinityMfma = 1;    // 1: B and W of the affinity on the matrix cores; 0: on the VALU
     int emCull = 1;            // 1: skip column groups whose affinities all underflow to 0
+    int emRowFuse = 1;         // 1: single-chunk tiles finalize their rows in the affinity kernel
     int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)
 };
 

@@ -65,6 +65,14 @@ int emsa_probe_hilbert_order(const float *dP, int n, int *dPerm);
  * For A/B measurement; drops the recorded launch sequences of the device. */
 int emsa_probe_set_affinity_cull(int enable);
 
+/* Selects, on the current device, where the EM-ICP row finalize runs when
+ * every tile of a launch spans its registration's whole target (one column
+ * chunk, e.g. emsa_set_tiling(0, tile_cols) with tile_cols >= N): 1 (default)
+ * in the affinity kernel's epilogue, no row-finalize launch; 0 in its own
+ * kernel after the partial sums went through HBM. Both give bit-identical
+ * results. For A/B measurement; drops the recorded launch sequences. */
+int emsa_probe_set_row_fuse(int enable);
+
 #ifdef __cplusplus
 }
 #endif
