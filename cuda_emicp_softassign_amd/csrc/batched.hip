@@ -275,8 +275,15 @@ int emsa_autotune_emicp(const emsa_pair* pairs, int count, int n_starts, const e
     std::lock_guard<std::mutex> lk(ctx->mu);
     hipStream_t s = (hipStream_t)stream;
     const bool single = (count == 1 && n_starts == 1);
-    const std::vector<int> cands = single ? std::vector<int>{2, 4, 8, 16, 32}
-                                          : std::vector<int>{512, 1024, 2048, 4096, 8192};
+    std::vector<int> cands = single ? std::vector<int>{2, 4, 8, 16, 32}
+                                    : std::vector<int>{512, 1024, 2048, 4096, 8192};
+    if (!single) {  // whole targets: one chunk each, rows finalized in the affinity kernel
+        int wmax = 0;
+        for (int k = 0; k < count; ++k)
+            wmax = std::max(wmax, (pairs[k].N + kEmWaves * kColGroup - 1) /
+                                      (kEmWaves * kColGroup) * (kEmWaves * kColGroup));
+        if (wmax > cands.back() && wmax <= 65536) cands.push_back(wmax);
+    }
     const size_t nregs = (size_t)count * n_starts;
     if ((rc = ctx->rt.reserve(sizeof(float) * 24 * nregs)) != EMSA_OK) return rc;
     std::vector<float> rt0(12 * nregs, 0.f);

@@ -27,7 +27,10 @@
 //     (C, B, W) go to HBM.
 //  2. emicp_rows_kernel: one wave per row block reduces the chunk partials in
 //     a fixed order in FP64, forms lambda_r and lambda_r X'_r (or the floored
-//     row) and the 18 FP64 moments of its 64 rows.
+//     row) and the 18 FP64 moments of its 64 rows. A registration whose tiles
+//     span its whole target (one chunk) has this done in the affinity
+//     kernel's epilogue instead, from the same sums (bit-identical), and the
+//     launch is dropped when every registration is such.
 //  3. emicp_solve_kernel: one workgroup per registration sums the row-block
 //     moments in a fixed order, forms Xc, Yc, S and solves Horn's problem in
 //     FP64, updating (R, t).
@@ -607,13 +610,14 @@ __device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, d
 // Stage 2: one wave per row block; moments to HBM (moment-major).
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
-                  double outlier, int last, int iter) {
+                  double outlier, int last, int iter, int skipSingle) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long gr = (long long)blockIdx.x * kEmWaves + wave;
     if (gr >= totalRowBlocks) return;  // wave-uniform
     const int lane = threadIdx.x & (kWave - 1);
     const int ri = find_reg_rows(regs, nregs, (int)gr);
     const EmRegDesc& d = regs[ri];
+    if (skipSingle && d.chunks == 1) return;  // finalized by the affinity kernel
     const int rb = (int)gr - d.rowBase;
     row_block_moments(d, rb, lane, outlier, last, iter);
 }
@@ -862,7 +866,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 #pragma unroll
             for (int w = 1; w < kEmWaves; ++w) acc[qq] += stage[w][0][qq * kEmRows + lane];
         }
-        if (FUSE) {
+        if (FUSE && d.chunks == 1) {
             // the tile spans the whole target (one chunk): these are the
             // row sums the row finalize would read back, so it runs here
             row_block_finalize<true>(d, rb, lane, (double)acc[0], (double)acc[1],
@@ -1050,7 +1054,11 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     w.tiles = tiles;
     w.rowBlocks = rowBlocks;
     w.singleChunk = true;
-    for (const EmRegDesc& d : w.regs) w.singleChunk = w.singleChunk && d.chunks == 1;
+    w.someSingleChunk = false;
+    for (const EmRegDesc& d : w.regs) {
+        w.singleChunk = w.singleChunk && d.chunks == 1;
+        w.someSingleChunk = w.someSingleChunk || d.chunks == 1;
+    }
     const int nregs = (int)w.regs.size();
     int rc;
     if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
@@ -1098,9 +1106,11 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     int rc = em_launch_setup(w, dRt0, s);
     if (rc != EMSA_OK) return rc;
     const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
-    // every tile spans its whole target: the affinity kernel finalizes the
-    // rows itself and the row kernel is not launched
-    const bool fuse = ctx.emRowFuse && ctx.emAffinityMfma && w.singleChunk;
+    // registrations whose tiles span their whole target (one chunk) have
+    // their rows finalized by the affinity kernel; the row kernel serves the
+    // others and is not launched when there are none
+    const bool fuse = ctx.emRowFuse && ctx.emAffinityMfma && w.someSingleChunk;
+    const bool rowsKernel = !(fuse && w.singleChunk);
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
@@ -1112,9 +1122,9 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
                         fuse ? &fz : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
-        if (!fuse) {
+        if (rowsKernel) {
             hipLaunchKernelGGL(emicp_rows_kernel, dim3(rowGrid), dim3(kEmRows * kEmWaves), 0, s,
-                               desc, nregs, w.rowBlocks, outlier, last, (int)it);
+                               desc, nregs, w.rowBlocks, outlier, last, (int)it, fuse ? 1 : 0);
             EMSA_HIP_TRY(hipGetLastError());
         }
         hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,

@@ -102,7 +102,8 @@ struct EmWork {
     std::vector<EmRegDesc> regs;
     long long tiles = 0;       // affinity workgroups per step
     long long rowBlocks = 0;   // row blocks per step over the whole table
-    bool singleChunk = false;  // every registration's tile spans its whole target
+    bool singleChunk = false;      // every registration's tile spans its whole target
+    bool someSingleChunk = false;  // some registration's tile spans its whole target
 };
 
 // What the affinity kernel needs to finalize the rows itself (one chunk per

@@ -235,11 +235,12 @@ def test_affinity_cull_non_finite_inputs(E):
 def test_row_fuse_bit_identical(E):
     """With tiles that span each registration's whole target (one column
     chunk), the affinity kernel finalizes the rows in its epilogue and the
-    row-finalize kernel is not launched. Same bits as the row kernel working
-    from the partial sums in HBM: batched multi-start runs of pairs of uneven
-    sizes (targets up to 4100 points, 4160-column tiles) and a traced plain
-    20,000-point registration (one workgroup per CU, so one chunk per row
-    block)."""
+    row-finalize kernel is not launched (or skips those registrations in a
+    mixed batch). Same bits as the row kernel working from the partial sums
+    in HBM: batched multi-start runs of pairs of uneven sizes with 2048-column
+    tiles (mixed) and 4160-column tiles (every target in one chunk), and a
+    traced plain 20,000-point registration (one workgroup per CU, so one chunk
+    per row block)."""
     from cuda_emicp_softassign_amd import _lib
     L = _lib.load()
     pairs = _pairs()
@@ -249,15 +250,17 @@ def test_row_fuse_bit_identical(E):
     X, Y, _, _ = uniform_cube(20000, seed=41)
     res = []
     try:
-        E.set_tiling(1, 4160)
         for fuse in (0, 1):
             _lib.check("fuse", L.emsa_probe_set_row_fuse(fuse))
+            E.set_tiling(0, 0)  # 2048-column tiles: targets of 60, 1007, 33 points
+            Rm, rm = E.emicp_batched(pairs, Rt0, p)  # one chunk, 2500 and 4100: more
+            E.set_tiling(1, 4160)
             Rb, rb = E.emicp_batched(pairs, Rt0, p)
             R, t, tr = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p)
-            res.append((Rb, rb, R, t, tr))
+            res.append((Rm, rm, Rb, rb, R, t, tr))
     finally:
         _lib.check("fuse", L.emsa_probe_set_row_fuse(1))
         E.set_tiling(0, 0)
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
-    assert len(res[1][4]) == 66 and not res[1][4][:, 31].any()
+    assert len(res[1][6]) == 66 and not res[1][6][:, 31].any()
