@@ -154,6 +154,19 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         xr[j][1] = b.X[(size_t)N + r];
         xr[j][2] = b.X[2 * (size_t)N + r];
     }
+    // log domain: the exponent k0 + g_c - k1 |x_r - p_c|^2 as
+    // a_r + b_c + x'_r . p_c with a_r = -k1 |x_r|^2, x'_r = 2 k1 x_r and
+    // b_c = k0 + g_c - k1 |p_c|^2 (one add and 3 FMA per element; the
+    // expansion is safe at Softassign temperatures, where k1 |x|^2 is O(1))
+    float ar[R], xq[R][3];
+    if (LOG) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            ar[j] = -k1 * fmaf(xr[j][0], xr[j][0], fmaf(xr[j][1], xr[j][1], xr[j][2] * xr[j][2]));
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xq[j][a] = 2.f * k1 * xr[j][a];
+        }
+    }
     float acc[R][Q];
 #pragma unroll
     for (int j = 0; j < R; ++j)
@@ -204,6 +217,12 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         if (MODE != 0) load4(b.v + c, vv);
 #pragma unroll
         for (int i = 0; i < 4; ++i) kg[i] = LOG ? (k0 + vv[i]) : k0;  // log: v holds g_c
+        float bc[4];
+        if (LOG)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                bc[i] = fmaf(-k1, fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i])),
+                             kg[i]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) w[i][0] = LOG ? 1.f : vv[i];
         if (MODE == 2) {  // precomputed by the last column finalize
@@ -222,6 +241,15 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             float val[4];
             if (!LOG && MODE != 0) {
                 load4(b.K + (size_t)r * ldk + c, val);
+            } else if (LOG) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float t = ar[j] + bc[i];
+                    t = fmaf(xq[j][0], p[0][i], t);
+                    t = fmaf(xq[j][1], p[1][i], t);
+                    t = fmaf(xq[j][2], p[2][i], t);
+                    val[i] = __builtin_amdgcn_exp2f(t);
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -233,9 +261,9 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
                     s = fmaf(dz, dz, s);
                     val[i] = __builtin_amdgcn_exp2f(fmaf(s, -k1, kg[i]));
                 }
-                if (!LOG)  // MODE 0 materialised: store K
-                    *reinterpret_cast<float4*>(b.K + (size_t)r * ldk + c) =
-                        make_float4(val[0], val[1], val[2], val[3]);
+                // MODE 0 materialised: store K
+                *reinterpret_cast<float4*>(b.K + (size_t)r * ldk + c) =
+                    make_float4(val[0], val[1], val[2], val[3]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -468,27 +496,34 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
     const int rs = h * chunkRows + wave * q;
     const int re = min(rs + q, N);
 
-    float p[3][4];
+    float p[3][4], pc[4];
     if (LOG) {
         load4(b.P + c, p[0]);
         load4(b.P + (size_t)ldk + c, p[1]);
         load4(b.P + 2 * (size_t)ldk + c, p[2]);
+        // the exponent k0 + f_r - k1 |x_r - p_c|^2 expanded as in the row
+        // passes: (k0 + f_r - k1 |x_r|^2) + (-k1 |p_c|^2) + 2 k1 x_r . p_c
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            pc[i] = -k1 * fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i]));
     }
+    const float k2 = 2.f * k1;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
     for (int r = rs; r < re; ++r) {
-        const float ur = b.u[r];  // log: f_r
         if (LOG) {
             const float x0 = b.X[r], x1 = b.X[(size_t)N + r], x2 = b.X[2 * (size_t)N + r];
-            const float kf = k0 + ur;
+            const float a = fmaf(-k1, fmaf(x0, x0, fmaf(x1, x1, x2 * x2)), k0 + b.u[r]);
+            const float q0 = k2 * x0, q1 = k2 * x1, q2 = k2 * x2;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float dx = x0 - p[0][i], dy = x1 - p[1][i], dz = x2 - p[2][i];
-                float s = dx * dx;
-                s = fmaf(dy, dy, s);
-                s = fmaf(dz, dz, s);
-                acc[i] += __builtin_amdgcn_exp2f(fmaf(s, -k1, kf));
+                float t = a + pc[i];
+                t = fmaf(q0, p[0][i], t);
+                t = fmaf(q1, p[1][i], t);
+                t = fmaf(q2, p[2][i], t);
+                acc[i] += __builtin_amdgcn_exp2f(t);
             }
         } else {
+            const float ur = b.u[r];
             float kv[4];
             load4(b.K + (size_t)r * ldk + c, kv);
 #pragma unroll
