@@ -2,9 +2,10 @@
 
 The reference has no CPU Softassign, so parity is against the FP64 oracle of
 the identical iteration (oracle/emsa_oracle_impl.h, literal materialised
-matrix with in-place divisions). Tolerance: R within 1e-4 rad per inner
-iteration and at the end, the per-iteration normaliser sum M within 1e-4
-relative, t within 1e-5 of the cloud extent.
+matrix with in-place divisions). Tolerances: the per-iteration normaliser
+sum M within 1e-5 relative (measured <= 2.3e-7 over these cases), R within
+2e-5 rad per inner iteration (measured <= 3.1e-6) and 1e-4 rad at the end,
+t within 1e-5 of the cloud extent.
 """
 import numpy as np
 import pytest
@@ -30,7 +31,7 @@ def _case(seed=11, n=300, m=280):
     return X.astype(np.float32), Y.astype(np.float32)
 
 
-def _compare(E, oracle, X, Y, variant, tol=1e-4, **kw):
+def _compare(E, oracle, X, Y, variant, tol=1e-4, step_tol=2e-5, norm_tol=1e-5, **kw):
     p = E.registrationParameters(**kw)
     Ro, to, tro = oracle.softassign(X.astype(np.float64), Y.astype(np.float64),
                                     JMAX=p.JMAX, I0=p.I0, I1=p.I1, alpha=p.alpha, T_0=p.T_0,
@@ -38,8 +39,8 @@ def _compare(E, oracle, X, Y, variant, tol=1e-4, **kw):
     R, t, tr = E.softassign_traced(X, Y, np.eye(3), np.zeros(3), p, variant=variant)
     assert len(tr) == len(tro) == p.JMAX * p.I0
     for k in range(len(tr)):
-        assert geodesic(tr[k, 17:26].reshape(3, 3), tro[k, 17:26].reshape(3, 3)) < tol, k
-        assert abs(tr[k, 1] - tro[k, 1]) <= 1e-4 * abs(tro[k, 1]), k
+        assert geodesic(tr[k, 17:26].reshape(3, 3), tro[k, 17:26].reshape(3, 3)) < step_tol, k
+        assert abs(tr[k, 1] - tro[k, 1]) <= norm_tol * abs(tro[k, 1]), k
     extent = np.ptp(X, axis=0).max()
     assert geodesic(R.astype(np.float64), Ro) < tol
     assert np.abs(t - to).max() < 1e-5 * extent
