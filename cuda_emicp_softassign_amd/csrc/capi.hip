@@ -247,8 +247,13 @@ static int softassign_host(const float* X, int N, const float* Y, int M, float* 
     EMSA_HIP_TRY(hipMemcpyAsync(ctx->Y.ptr, Y, sizeof(float) * 3 * (size_t)M,
                                 hipMemcpyHostToDevice, s));
     EMSA_HIP_TRY(hipMemcpyAsync(drt, rt, sizeof(rt), hipMemcpyHostToDevice, s));
+    double xmax2 = 0.0;  // max |x|^2 of the target
+    for (int r = 0; r < N; ++r) {
+        const double a = X[r], b = X[(size_t)N + r], c = X[2 * (size_t)N + r];
+        xmax2 = std::max(xmax2, a * a + b * b + c * c);
+    }
     rc = softassign_device(*ctx, ctx->X.as<float>(), N, ctx->Y.as<float>(), M, drt, drt + 16, p,
-                           variant, s, htrace ? ctx->trace.as<double>() : nullptr);
+                           variant, s, htrace ? ctx->trace.as<double>() : nullptr, (float)xmax2);
     if (rc != EMSA_OK) return rc;
     float out[12];
     EMSA_HIP_TRY(hipMemcpyAsync(out, drt + 16, sizeof(out), hipMemcpyDeviceToHost, s));

@@ -208,10 +208,11 @@ int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                  const float* dRtIn, float* dRtOut, const emsa_params& p,
                  const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows);
 
-// Softassign on device buffers (see softassign.hip).
+// Softassign on device buffers (see softassign.hip). xmax2: max |x|^2 over
+// the target cloud (chooses the log domain's form of the exponent).
 int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                       const float* dRtIn, float* dRtOut, const emsa_params& p, int variant,
-                      hipStream_t s, double* trace);
+                      hipStream_t s, double* trace, float xmax2);
 
 // Horn Procrustes on the device for one (Xc, Yc, S) (all float in, float out).
 int find_rt_device(DeviceCtx& ctx, const float* hXc, const float* hYc, const float* hS,

@@ -53,6 +53,12 @@ constexpr int kSaColsPB = 256;               // columns per column-pass workgrou
 constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
 constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
 constexpr float kSaSentinel = 1.0e15f;
+// The log domain evaluates the exponent expanded (a_r + b_c + x'_r . p_c)
+// at the temperatures where k1 max_r |x_r|^2 is at most this: the expanded
+// terms then round to about 1e-5 of the exponent at worst. Where the
+// affinity is not negligible p_c lies near some x_r, so the target cloud's
+// bound covers |p_c|^2 there. Colder steps use the direct difference.
+constexpr float kSaExpandMax = 64.f;
 
 struct SaPlan {
     int ldk = 0;        // padded column count (multiple of 256)
@@ -137,7 +143,7 @@ __device__ inline void load4(const float* p, float (&v)[4]) {
 // final pass (s_r, D'_r and the workgroup's FP64 moments).
 // ---------------------------------------------------------------------------
 
-template <bool LOG, int MODE, int R = kSaRowsWG, int NW = kSaWaves>
+template <bool LOG, int MODE, int R = kSaRowsWG, int NW = kSaWaves, bool EXP = false>
 __global__ void __launch_bounds__(kWave * NW)
 sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
     constexpr int Q = (MODE == 2) ? 4 : 1;
@@ -154,12 +160,11 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         xr[j][1] = b.X[(size_t)N + r];
         xr[j][2] = b.X[2 * (size_t)N + r];
     }
-    // log domain: the exponent k0 + g_c - k1 |x_r - p_c|^2 as
+    // log domain, EXP: the exponent k0 + g_c - k1 |x_r - p_c|^2 as
     // a_r + b_c + x'_r . p_c with a_r = -k1 |x_r|^2, x'_r = 2 k1 x_r and
-    // b_c = k0 + g_c - k1 |p_c|^2 (one add and 3 FMA per element; the
-    // expansion is safe at Softassign temperatures, where k1 |x|^2 is O(1))
+    // b_c = k0 + g_c - k1 |p_c|^2 (one add and 3 FMA per element)
     float ar[R], xq[R][3];
-    if (LOG) {
+    if (LOG && EXP) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             ar[j] = -k1 * fmaf(xr[j][0], xr[j][0], fmaf(xr[j][1], xr[j][1], xr[j][2] * xr[j][2]));
@@ -218,7 +223,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) kg[i] = LOG ? (k0 + vv[i]) : k0;  // log: v holds g_c
         float bc[4];
-        if (LOG)
+        if (LOG && EXP)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 bc[i] = fmaf(-k1, fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i])),
@@ -241,7 +246,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             float val[4];
             if (!LOG && MODE != 0) {
                 load4(b.K + (size_t)r * ldk + c, val);
-            } else if (LOG) {
+            } else if (LOG && EXP) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     float t = ar[j] + bc[i];
@@ -261,9 +266,9 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
                     s = fmaf(dz, dz, s);
                     val[i] = __builtin_amdgcn_exp2f(fmaf(s, -k1, kg[i]));
                 }
-                // MODE 0 materialised: store K
-                *reinterpret_cast<float4*>(b.K + (size_t)r * ldk + c) =
-                    make_float4(val[0], val[1], val[2], val[3]);
+                if (!LOG)  // MODE 0 materialised: store K
+                    *reinterpret_cast<float4*>(b.K + (size_t)r * ldk + c) =
+                        make_float4(val[0], val[1], val[2], val[3]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -484,7 +489,7 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
 // Column pass, stage 1: a workgroup = 256 columns x one row chunk; lanes own
 // 4 consecutive columns, the 4 waves split the chunk's rows.
 // ---------------------------------------------------------------------------
-template <bool LOG>
+template <bool LOG, bool EXP = false>
 __global__ void __launch_bounds__(kWave * kSaWaves)
 sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k0) {
     const int cbk = blockIdx.x / H;
@@ -501,16 +506,17 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
         load4(b.P + c, p[0]);
         load4(b.P + (size_t)ldk + c, p[1]);
         load4(b.P + 2 * (size_t)ldk + c, p[2]);
-        // the exponent k0 + f_r - k1 |x_r - p_c|^2 expanded as in the row
-        // passes: (k0 + f_r - k1 |x_r|^2) + (-k1 |p_c|^2) + 2 k1 x_r . p_c
+        // EXP: the exponent k0 + f_r - k1 |x_r - p_c|^2 expanded as in the
+        // row passes: (k0 + f_r - k1 |x_r|^2) + (-k1 |p_c|^2) + 2 k1 x_r . p_c
+        if (EXP)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            pc[i] = -k1 * fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i]));
+            for (int i = 0; i < 4; ++i)
+                pc[i] = -k1 * fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i]));
     }
     const float k2 = 2.f * k1;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
     for (int r = rs; r < re; ++r) {
-        if (LOG) {
+        if (LOG && EXP) {
             const float x0 = b.X[r], x1 = b.X[(size_t)N + r], x2 = b.X[2 * (size_t)N + r];
             const float a = fmaf(-k1, fmaf(x0, x0, fmaf(x1, x1, x2 * x2)), k0 + b.u[r]);
             const float q0 = k2 * x0, q1 = k2 * x1, q2 = k2 * x2;
@@ -521,6 +527,17 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
                 t = fmaf(q1, p[1][i], t);
                 t = fmaf(q2, p[2][i], t);
                 acc[i] += __builtin_amdgcn_exp2f(t);
+            }
+        } else if (LOG) {
+            const float x0 = b.X[r], x1 = b.X[(size_t)N + r], x2 = b.X[2 * (size_t)N + r];
+            const float kf = k0 + b.u[r];  // f_r
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float dx = x0 - p[0][i], dy = x1 - p[1][i], dz = x2 - p[2][i];
+                float s = dx * dx;
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                acc[i] += __builtin_amdgcn_exp2f(fmaf(s, -k1, kf));
             }
         } else {
             const float ur = b.u[r];
@@ -760,6 +777,21 @@ bool launch_rowcol_upto(int slabs, bool first, dim3 grid, dim3 blk, hipStream_t 
     return false;
 }
 
+// Log-domain row pass of the given mode, expanded exponent or direct difference.
+template <bool EXP>
+void launch_log_rows(int mode, dim3 grid, dim3 blk, hipStream_t s, const SaBufs& b, int N, int M,
+                     int ldk, float k1, float k0) {
+    if (mode == 0)
+        hipLaunchKernelGGL((sa_rows_kernel<true, 0, kSaRowsWG, kSaWaves, EXP>), grid, blk, 0, s, b,
+                           N, M, ldk, k1, k0);
+    else if (mode == 1)
+        hipLaunchKernelGGL((sa_rows_kernel<true, 1, kSaRowsWG, kSaWaves, EXP>), grid, blk, 0, s, b,
+                           N, M, ldk, k1, k0);
+    else
+        hipLaunchKernelGGL((sa_rows_kernel<true, 2, kSaRowsWG, kSaWaves, EXP>), grid, blk, 0, s, b,
+                           N, M, ldk, k1, k0);
+}
+
 int launch_rowcol(bool first, int slabs, dim3 grid, dim3 blk, hipStream_t s, const SaBufs& b,
                   int N, int M, int ldk, float k1, float k0) {
     if (!launch_rowcol_upto<1>(slabs, first, grid, blk, s, b, N, M, ldk, k1, k0))
@@ -772,7 +804,7 @@ int launch_rowcol(bool first, int slabs, dim3 grid, dim3 blk, hipStream_t s, con
 
 int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                       const float* dRtIn, float* dRtOut, const emsa_params& p, int variant,
-                      hipStream_t s, double* trace) {
+                      hipStream_t s, double* trace, float xmax2) {
     const bool LOGV = (variant == EMSA_SOFTASSIGN_LOGDOMAIN);
     const SaPlan pl = plan_sa(N, M, ctx.numCU);
     // moment rows of the final pass (one per final-pass workgroup)
@@ -836,18 +868,22 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         const double k1d = kLog2e / (double)T;
         const float k1 = (float)k1d;
         const float k0 = (float)((double)p.alpha * k1d - 0.5 * std::log2((double)T));
+        const bool expand = k1 * xmax2 <= kSaExpandMax;  // log domain: the exponent's form
+        auto log_rows = [&](int mode) {
+            if (expand) launch_log_rows<true>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
+            else launch_log_rows<false>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
+        };
         for (int i0 = 0; i0 < p.I0; ++i0, ++it) {
             for (int sw = 0; sw < p.I1; ++sw) {
                 const int last = (sw + 1 == p.I1) ? 1 : 0;
                 if (LOGV) {
-                    if (sw == 0)
-                        hipLaunchKernelGGL((sa_rows_kernel<true, 0>), rgrid, blk, 0, s, b, N, M,
-                                           pl.ldk, k1, k0);
+                    log_rows(sw == 0 ? 0 : 1);
+                    if (expand)
+                        hipLaunchKernelGGL((sa_cols_kernel<true, true>), cgrid, blk, 0, s, b, N,
+                                           pl.ldk, pl.H, pl.chunkRows, k1, k0);
                     else
-                        hipLaunchKernelGGL((sa_rows_kernel<true, 1>), rgrid, blk, 0, s, b, N, M,
-                                           pl.ldk, k1, k0);
-                    hipLaunchKernelGGL((sa_cols_kernel<true>), cgrid, blk, 0, s, b, N, pl.ldk,
-                                       pl.H, pl.chunkRows, k1, k0);
+                        hipLaunchKernelGGL((sa_cols_kernel<true, false>), cgrid, blk, 0, s, b, N,
+                                           pl.ldk, pl.H, pl.chunkRows, k1, k0);
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_colfin_kernel<true, true>), fgrid, blk, 0, s, b, M,
                                            pl.ldk, pl.H, last);
@@ -885,8 +921,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                 EMSA_HIP_TRY(hipGetLastError());
             }
             if (LOGV)
-                hipLaunchKernelGGL((sa_rows_kernel<true, 2>), rgrid, blk, 0, s, b, N, M, pl.ldk,
-                                   k1, k0);
+                log_rows(2);
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2, kSaFinalRows, kSaFinalWaves>),
                                    dim3((unsigned)finalBlocks), dim3(kWave * kSaFinalWaves), 0, s,

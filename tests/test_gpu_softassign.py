@@ -80,6 +80,17 @@ def test_ragged_sizes(E, oracle, variant, N, M):
              I1=2, T_0=1.0, TFACTOR=0.9)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_cold_schedule(E, oracle, variant):
+    """T from 0.05 down to 0.0024: k1 max |x|^2 passes 64 after 12 of the
+    30 temperatures, so the log domain evaluates the exponent expanded first
+    and as the direct difference after; both variants match the oracle
+    throughout."""
+    X, Y = _case(seed=8, n=200, m=180)
+    _compare(E, oracle, X, Y, variant, JMAX=30, I0=5, I1=3, T_0=0.05, TFACTOR=0.9,
+             alpha=0.05)
+
+
 def test_variants_agree_full_size(E):
     """N = M = 5032 surface + 10% outliers, reference defaults: the
     materialised and log-domain variants give the same registration."""
