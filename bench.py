@@ -42,9 +42,11 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
 # (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
 # FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
-# WRITE_SIZE, run on `bench.py --counters affinity`; 15.6 MB fetched (each of
-# the 8 L2s reads the ~1.9 MB of inputs) + 8.3 MB of partial sums written
-AFFINITY_TRAFFIC_BYTES = 24.0e6
+# WRITE_SIZE, run on `bench.py --counters affinity`: the plain-difference
+# instance fetches 15.6 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and
+# writes 8.4 MB of partial sums; the compensated one adds ~20 MB of register
+# spill traffic (sequence average over its 66 launches)
+AFFINITY_TRAFFIC_BYTES = 39.2e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
