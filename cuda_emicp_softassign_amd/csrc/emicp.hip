@@ -485,17 +485,16 @@ emicp_affinity_kernel(const EmRegDesc* __restrict__ regs, int nregs, float negk)
 // Row finalize of row rb * 64 + lane from its sums over all target columns
 // (C without the outlier term, B, W): the floor, lambda, lambda X', the
 // per-row outputs and checkpoint; then the 18 FP64 moments of the row block,
-// each summed over the wave as it is formed (one live at a time) and stored
-// by lane q (moment-major).
+// each summed over the 64 rows in row order by lane q from the rows' terms in
+// LDS (t: this wave's kRowTerms x 64 doubles), and stored by lane q
+// (moment-major). Moment q is the product of terms kMomA[q] and kMomB[q]:
 //  [0] lambda  [1..3] lambda X'  [4..6] lambda y  [7 + 3i + j] lambda X'_i y_j
 //  [16] lambda |X' - (R y + t)|^2  [17] floored rows
-// SERIAL: one moment formed and summed at a time (few registers, for a
-// kernel held to 8 waves per SIMD); otherwise all 18 sums are interleaved.
-template <bool SERIAL>
-__device__ __attribute__((always_inline)) inline void row_block_finalize(const EmRegDesc& d, int rb, int lane, double Cs,
-                                          double Bs, double Xs, double Ys, double Zs,
-                                          double outlier, int last, int iter) {
-    const int N = d.N, M = d.M;
+// terms: 0 lambda, 1..3 lambda X', 4..6 y, 7 residual, 8 floored, 9 one
+constexpr int kRowTerms = 10;
+__device__ __attribute__((always_inline)) inline void row_block_finalize(
+    const EmRegDesc& d, int rb, int lane, double Cs, double Bs, double Xs, double Ys, double Zs,
+    double outlier, int last, int iter, double* __restrict__ t) {    const int N = d.N, M = d.M;
     const int r = rb * kEmRows + lane;
     const bool valid = r < M;
     const double* __restrict__ state = d.state;
@@ -543,40 +542,38 @@ __device__ __attribute__((always_inline)) inline void row_block_finalize(const E
         const double ex = Vx / lam - pxd, ey = Vy / lam - pyd, ez = Vz / lam - pzd;
         res = lam * (ex * ex + ey * ey + ez * ez);
     }
-    const double y0 = yx, y1 = yy, y2 = yz;
-    double mine = 0.0;  // moment `lane` of the row block
-    auto emit = [&](int q, double v) {
-        v = wave_sum(v);
-        if (lane == q) mine = v;
-        // serial: keep the scheduler from interleaving the 18 sums
-        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
-    };
-    emit(0, lam);
-    emit(1, Vx);
-    emit(2, Vy);
-    emit(3, Vz);
-    emit(4, lam * y0);
-    emit(5, lam * y1);
-    emit(6, lam * y2);
-    emit(7, Vx * y0);
-    emit(8, Vx * y1);
-    emit(9, Vx * y2);
-    emit(10, Vy * y0);
-    emit(11, Vy * y1);
-    emit(12, Vy * y2);
-    emit(13, Vz * y0);
-    emit(14, Vz * y1);
-    emit(15, Vz * y2);
-    emit(16, res);
-    emit(17, fl);
+    t[0 * kEmRows + lane] = lam;
+    t[1 * kEmRows + lane] = Vx;
+    t[2 * kEmRows + lane] = Vy;
+    t[3 * kEmRows + lane] = Vz;
+    t[4 * kEmRows + lane] = (double)yx;
+    t[5 * kEmRows + lane] = (double)yy;
+    t[6 * kEmRows + lane] = (double)yz;
+    t[7 * kEmRows + lane] = res;
+    t[8 * kEmRows + lane] = fl;
+    t[9 * kEmRows + lane] = 1.0;
+    wave_lds_sync();
     static_assert(kMomW == 18, "moment layout above");
-    if (lane < kMomW) d.moments[(size_t)lane * d.rowBlocks + rb] = mine;
+    if (lane < kMomW) {
+        const int q = lane;
+        int ia, ib;
+        if (q < 4) { ia = q; ib = 9; }
+        else if (q < 7) { ia = 0; ib = q; }
+        else if (q < 16) { ia = 1 + (q - 7) / 3; ib = 4 + (q - 7) % 3; }
+        else { ia = q - 9; ib = 9; }
+        const double* __restrict__ ta = t + ia * kEmRows;
+        const double* __restrict__ tb = t + ib * kEmRows;
+        double sum = 0.0;
+#pragma unroll 8
+        for (int rr = 0; rr < kEmRows; ++rr) sum = fma(ta[rr], tb[rr], sum);
+        d.moments[(size_t)q * d.rowBlocks + rb] = sum;
+    }
 }
 
 // The row block's sums from the chunk partials (in chunk order, FP64; the
 // loads of 4 chunks are issued before their adds), then the finalize.
 __device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, double outlier,
-                                         int last, int iter) {
+                                         int last, int iter, double* __restrict__ t) {
     const int chunks = d.chunks;
     double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
     const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
@@ -604,7 +601,7 @@ __device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, d
         Ys += q[3 * kEmRows];
         Zs += q[4 * kEmRows];
     }
-    row_block_finalize<false>(d, rb, lane, Cs, Bs, Xs, Ys, Zs, outlier, last, iter);
+    row_block_finalize(d, rb, lane, Cs, Bs, Xs, Ys, Zs, outlier, last, iter, t);
 }
 
 // Stage 2: one wave per row block; moments to HBM (moment-major).
@@ -619,7 +616,8 @@ emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long total
     const EmRegDesc& d = regs[ri];
     if (skipSingle && d.chunks == 1) return;  // finalized by the affinity kernel
     const int rb = (int)gr - d.rowBase;
-    row_block_moments(d, rb, lane, outlier, last, iter);
+    __shared__ double terms[kEmWaves][kRowTerms * kEmRows];
+    row_block_moments(d, rb, lane, outlier, last, iter, terms[wave]);
 }
 
 // Stage 3 for registration d from its 18 summed moments (one thread):
@@ -869,9 +867,14 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         if (FUSE && d.chunks == 1) {
             // the tile spans the whole target (one chunk): these are the
             // row sums the row finalize would read back, so it runs here
-            row_block_finalize<true>(d, rb, lane, (double)acc[0], (double)acc[1],
-                                     (double)acc[2], (double)acc[3], (double)acc[4], outlier,
-                                     last, iter);
+            // the stage is free again: wave 0 has read it and the other
+            // waves are done with it (the barrier above)
+            static_assert(kRowTerms * kEmRows * sizeof(double) <= sizeof(stage),
+                          "row terms fit the stage");
+            wave_lds_sync();
+            row_block_finalize(d, rb, lane, (double)acc[0], (double)acc[1], (double)acc[2],
+                               (double)acc[3], (double)acc[4], outlier, last, iter,
+                               reinterpret_cast<double*>(&stage[0][0][0]));
         } else {
             float* P = d.partial + (size_t)local * 5 * kEmRows;
 #pragma unroll
