@@ -74,9 +74,24 @@ def test_multistart_finds_large_rotation(E, pq_pairs):
     assert geodesic(R1.astype(float), Rg) > 0.1  # the single start does not get there
 
 
-def test_graph_replays_match_eager(E):
+def _small_pairs():
+    """Targets of at most 2048 points: with 2048-column tiles every
+    registration is one chunk, so no row-finalize kernel is launched."""
+    out = []
+    for k, (n, m) in enumerate([(60, 1500), (2000, 900), (1007, 1007), (33, 17)]):
+        X, Y, R, t = range_scans(n, m, seed=200 + k)
+        out.append((X, Y))
+    return out
+
+
+@pytest.mark.parametrize("which", ["mixed", "single_chunk"])
+def test_graph_replays_match_eager(E, which):
+    """Replays of the recorded sequence equal the eager run bit for bit, with
+    eager work between replays; "mixed": some registrations finalize their
+    rows in the affinity kernel, the others in the row kernel; "single_chunk":
+    all in the affinity kernel, no row kernel in the graph."""
     import torch
-    pairs = _pairs()
+    pairs = _pairs() if which == "mixed" else _small_pairs()
     starts = [np.eye(3), rotation([0, 1, 0], 0.25)]
     Rt0 = np.stack([E.multistart_rt0(X, Y, starts) for X, Y in pairs]).astype(np.float32)
     p = E.registrationParameters(sigma_inf=2e-4)
