@@ -570,54 +570,53 @@ __device__ __attribute__((always_inline)) inline void row_block_finalize(
     }
 }
 
-// The row block's sums from the chunk partials (in chunk order, FP64; the
-// loads of 4 chunks are issued before their adds), then the finalize.
-__device__ inline void row_block_moments(const EmRegDesc& d, int rb, int lane, double outlier,
-                                         int last, int iter, double* __restrict__ t) {
-    const int chunks = d.chunks;
-    double Cs = 0.0, Bs = 0.0, Xs = 0.0, Ys = 0.0, Zs = 0.0;
-    const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
-    int k = 0;
-    for (; k + 4 <= chunks; k += 4) {
-        float v[4][5];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int q = 0; q < 5; ++q) v[i][q] = Pr[((size_t)(k + i) * 5 + q) * kEmRows];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            Cs += v[i][0];
-            Bs += v[i][1];
-            Xs += v[i][2];
-            Ys += v[i][3];
-            Zs += v[i][4];
-        }
-    }
-    for (; k < chunks; ++k) {
-        const float* q = Pr + (size_t)k * 5 * kEmRows;
-        Cs += q[0];
-        Bs += q[kEmRows];
-        Xs += q[2 * kEmRows];
-        Ys += q[3 * kEmRows];
-        Zs += q[4 * kEmRows];
-    }
-    row_block_finalize(d, rb, lane, Cs, Bs, Xs, Ys, Zs, outlier, last, iter, t);
-}
-
-// Stage 2: one wave per row block; moments to HBM (moment-major).
+// Stage 2: one workgroup per row block. Wave w sums the chunk partials
+// k = w, w + 4, ... in FP64 (in chunk order; the loads of 4 chunks are issued
+// before their adds), the 4 waves' sums are added in a fixed order in LDS,
+// then wave 0 finalizes the rows and stores the moments (moment-major). With
+// one chunk only wave 0 has a partial and the others add +0.0, so the sums
+// equal the affinity kernel's own (the fused finalize) bit for bit.
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
 emicp_rows_kernel(const EmRegDesc* __restrict__ regs, int nregs, long long totalRowBlocks,
                   double outlier, int last, int iter, int skipSingle) {
+    const long long gr = blockIdx.x;
+    if (gr >= totalRowBlocks) return;  // workgroup-uniform
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const long long gr = (long long)blockIdx.x * kEmWaves + wave;
-    if (gr >= totalRowBlocks) return;  // wave-uniform
     const int lane = threadIdx.x & (kWave - 1);
     const int ri = find_reg_rows(regs, nregs, (int)gr);
     const EmRegDesc& d = regs[ri];
     if (skipSingle && d.chunks == 1) return;  // finalized by the affinity kernel
     const int rb = (int)gr - d.rowBase;
-    __shared__ double terms[kEmWaves][kRowTerms * kEmRows];
-    row_block_moments(d, rb, lane, outlier, last, iter, terms[wave]);
+    const int chunks = d.chunks;
+    double v5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const float* Pr = d.partial + (size_t)rb * chunks * 5 * kEmRows + lane;
+    int k = wave;
+    for (; k + 3 * kEmWaves < chunks; k += 4 * kEmWaves) {
+        float v[4][5];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v[i][q] = Pr[((size_t)(k + i * kEmWaves) * 5 + q) * kEmRows];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v5[q] += v[i][q];
+    }
+    for (; k < chunks; k += kEmWaves) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) v5[q] += Pr[((size_t)k * 5 + q) * kEmRows];
+    }
+    __shared__ double part[kEmWaves][5][kEmRows];
+    __shared__ double terms[kRowTerms * kEmRows];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) part[wave][q][lane] = v5[q];
+    __syncthreads();
+    if (wave != 0) return;
+    double sm[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        sm[q] = ((part[0][q][lane] + part[1][q][lane]) + part[2][q][lane]) + part[3][q][lane];
+    row_block_finalize(d, rb, lane, sm[0], sm[1], sm[2], sm[3], sm[4], outlier, last, iter, terms);
 }
 
 // Stage 3 for registration d from its 18 summed moments (one thread):
@@ -1108,7 +1107,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     const EmRegDesc* desc = w.desc.as<EmRegDesc>();
     int rc = em_launch_setup(w, dRt0, s);
     if (rc != EMSA_OK) return rc;
-    const unsigned rowGrid = (unsigned)((w.rowBlocks + kEmWaves - 1) / kEmWaves);
+    const unsigned rowGrid = (unsigned)w.rowBlocks;  // one workgroup per row block
     // registrations whose tiles span their whole target (one chunk) have
     // their rows finalized by the affinity kernel; the row kernel serves the
     // others and is not launched when there are none
