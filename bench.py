@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # instance fetches 15.6 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and
 # writes 8.4 MB of partial sums; the compensated one adds ~20 MB of register
 # spill traffic (sequence average over its 66 launches)
-AFFINITY_TRAFFIC_BYTES = 39.2e6
+AFFINITY_TRAFFIC_BYTES = 39.3e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
