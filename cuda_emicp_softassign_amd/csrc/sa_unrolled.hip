@@ -56,6 +56,11 @@ constexpr int kFCols = 1024;           // columns per final-pass workgroup (4 wa
 constexpr int kStageW = 3 * kColGroup;
 constexpr int kStageLd = kWave + 1;
 constexpr float kSentinel = 1.0e15f;
+// The row and column passes evaluate the exponent expanded
+// (a_r + b_c + x'_r . p_c) at the temperatures where k1 max |x|^2 over the
+// batch's target clouds is at most this (the expanded terms then round to
+// about 1e-5 of the exponent at worst), the direct difference below.
+constexpr float kExpandMax = 64.f;
 typedef const __attribute__((address_space(4))) float* cfloat_p;
 
 struct Sau {
@@ -211,7 +216,7 @@ __global__ void __launch_bounds__(256) sau_transform_kernel(Sau d, int it) {
 // ---------------------------------------------------------------------------
 enum { kRowFwd = 0, kRowFinal = 1, kRowBwd = 2 };
 
-template <int MODE>
+template <int MODE, bool EXP>
 __global__ void __launch_bounds__(kWave * kWG)
 sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
     constexpr int NW = (MODE == kRowFinal) ? 4 : 1;
@@ -236,6 +241,20 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
         xr[j][1] = X[(size_t)N + r];
         xr[j][2] = X[2 * (size_t)N + r];
     }
+    // EXP: the exponent k0 - k1 |x_r - p_c|^2 as a_r + b_c + x'_r . p_c with
+    // a_r = k0 - k1 |x_r|^2, x'_r = 2 k1 x_r, b_c = -k1 |p_c|^2 (one add and
+    // 3 FMA per element)
+    float ar[kRW], xq[kRW][3];
+    if (EXP) {
+#pragma unroll
+        for (int j = 0; j < kRW; ++j) {
+            ar[j] = fmaf(-k1,
+                         fmaf(xr[j][0], xr[j][0], fmaf(xr[j][1], xr[j][1], xr[j][2] * xr[j][2])),
+                         k0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xq[j][a] = 2.f * k1 * xr[j][a];
+        }
+    }
     float acc[kRW][NW];
 #pragma unroll
     for (int j = 0; j < kRW; ++j)
@@ -243,10 +262,14 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
         for (int q = 0; q < NW; ++q) acc[j][q] = 0.f;
 
     for (int c = 4 * lane; c < ldk; c += 4 * kWave) {
-        float p[3][4], w[4][NW];
+        float p[3][4], w[4][NW], bc[4];
         load4(P + c, p[0]);
         load4(P + (size_t)ldk + c, p[1]);
         load4(P + 2 * (size_t)ldk + c, p[2]);
+        if (EXP)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                bc[i] = -k1 * fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i]));
         if (MODE == kRowFwd) {
             if (ones) {
 #pragma unroll
@@ -276,13 +299,22 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
             if (r0 + j >= N) continue;  // wave-uniform
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float dx = xr[j][0] - p[0][i];
-                const float dy = xr[j][1] - p[1][i];
-                const float dz = xr[j][2] - p[2][i];
-                float s = dx * dx;
-                s = fmaf(dy, dy, s);
-                s = fmaf(dz, dz, s);
-                const float K = __builtin_amdgcn_exp2f(fmaf(s, -k1, k0));
+                float t;
+                if (EXP) {
+                    t = ar[j] + bc[i];
+                    t = fmaf(xq[j][0], p[0][i], t);
+                    t = fmaf(xq[j][1], p[1][i], t);
+                    t = fmaf(xq[j][2], p[2][i], t);
+                } else {
+                    const float dx = xr[j][0] - p[0][i];
+                    const float dy = xr[j][1] - p[1][i];
+                    const float dz = xr[j][2] - p[2][i];
+                    float s = dx * dx;
+                    s = fmaf(dy, dy, s);
+                    s = fmaf(dz, dz, s);
+                    t = fmaf(s, -k1, k0);
+                }
+                const float K = __builtin_amdgcn_exp2f(t);
 #pragma unroll
                 for (int q = 0; q < NW; ++q) acc[j][q] = fmaf(K, w[i][q], acc[j][q]);
             }
@@ -393,7 +425,7 @@ sau_rowpass_kernel(Sau d, int k, int it, float k1, float k0) {
 // consecutive columns, the 4 waves split the chunk's rows. NW row weights
 // z[w][r] (w < NW) -> cpart[b][h][w][c].
 // ---------------------------------------------------------------------------
-template <int NW>
+template <int NW, bool EXP>
 __global__ void __launch_bounds__(kWave * kWG)
 sau_colpass_kernel(Sau d, const float* __restrict__ z, size_t zb, float k1, float k0) {
     const int b = blockIdx.y;
@@ -409,10 +441,17 @@ sau_colpass_kernel(Sau d, const float* __restrict__ z, size_t zb, float k1, floa
     const float* X = d.X + (size_t)b * 3 * N;
     const float* P = d.P + (size_t)b * 3 * ldk;
     const float* zz = z + (size_t)b * zb;
-    float p[3][4];
+    float p[3][4], bc[4];
     load4(P + c, p[0]);
     load4(P + (size_t)ldk + c, p[1]);
     load4(P + 2 * (size_t)ldk + c, p[2]);
+    // EXP: the exponent expanded as in the row passes:
+    // (k0 - k1 |x_r|^2) + b_c + 2 k1 x_r . p_c
+    if (EXP)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            bc[i] = -k1 * fmaf(p[0][i], p[0][i], fmaf(p[1][i], p[1][i], p[2][i] * p[2][i]));
+    const float k2 = 2.f * k1;
     float acc[NW][4];
 #pragma unroll
     for (int w = 0; w < NW; ++w)
@@ -423,13 +462,29 @@ sau_colpass_kernel(Sau d, const float* __restrict__ z, size_t zb, float k1, floa
     const cfloat_p Xs = (cfloat_p)X;
     const cfloat_p Zs = (cfloat_p)zz;
     auto pair4 = [&](float x0, float x1, float x2, const float (&zw)[NW]) {
+        float a = 0.f, q0 = 0.f, q1 = 0.f, q2 = 0.f;
+        if (EXP) {
+            a = fmaf(-k1, fmaf(x0, x0, fmaf(x1, x1, x2 * x2)), k0);
+            q0 = k2 * x0;
+            q1 = k2 * x1;
+            q2 = k2 * x2;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float dx = x0 - p[0][i], dy = x1 - p[1][i], dz = x2 - p[2][i];
-            float s = dx * dx;
-            s = fmaf(dy, dy, s);
-            s = fmaf(dz, dz, s);
-            const float K = __builtin_amdgcn_exp2f(fmaf(s, -k1, k0));
+            float t;
+            if (EXP) {
+                t = a + bc[i];
+                t = fmaf(q0, p[0][i], t);
+                t = fmaf(q1, p[1][i], t);
+                t = fmaf(q2, p[2][i], t);
+            } else {
+                const float dx = x0 - p[0][i], dy = x1 - p[1][i], dz = x2 - p[2][i];
+                float s = dx * dx;
+                s = fmaf(dy, dy, s);
+                s = fmaf(dz, dz, s);
+                t = fmaf(s, -k1, k0);
+            }
+            const float K = __builtin_amdgcn_exp2f(t);
 #pragma unroll
             for (int w = 0; w < NW; ++w) acc[w][i] = fmaf(K, zw[w], acc[w][i]);
         }
@@ -1081,12 +1136,62 @@ sau_adj_output_kernel(Sau d, float* __restrict__ gRt0, double* __restrict__ gT0,
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// max over the batch's target clouds of |x|^2 into out[0] (one workgroup;
+// the values are non-negative, so their bit patterns order as the floats)
+__global__ void __launch_bounds__(1024) sau_xmax_kernel(const float* __restrict__ X, int B,
+                                                        int N, float* __restrict__ out) {
+    float m = 0.f;
+    const long long total = (long long)B * N;
+    for (long long i = threadIdx.x; i < total; i += blockDim.x) {
+        const long long b = i / N, r = i - b * N;
+        const float* x = X + (size_t)b * 3 * N;
+        const float x0 = x[r], x1 = x[(size_t)N + r], x2 = x[2 * (size_t)N + r];
+        m = fmaxf(m, fmaf(x0, x0, fmaf(x1, x1, x2 * x2)));
+    }
+    m = wave_max(m);
+    __shared__ float wm[1024 / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = wm[0];
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) t = fmaxf(t, wm[w]);
+        out[0] = t;
+    }
+}
+
 struct SauWork {
+    DevBuf xmax;
     DevBuf P, state, u, v, orow, ocol, bcol, wfin, cpart, rmom, cby;
     DevBuf gU, gV, gOr, gOc, L, Q, zr, zc, adj, fcol, frow, fmomr, fmomc;
 };
 constexpr int kMaxDevices = 64;
 SauWork* g_sau[kMaxDevices];  // per device, created under that device's lock, never freed
+
+// max |x|^2 over the batch's target clouds, read back to the host (it picks
+// the exponent's form per temperature)
+int sau_xmax2(SauWork& w, const float* dX, int B, int N, hipStream_t s, float& out) {
+    int rc;
+    if ((rc = w.xmax.reserve(sizeof(float) * 64)) != EMSA_OK) return rc;
+    hipLaunchKernelGGL(sau_xmax_kernel, dim3(1), dim3(1024), 0, s, dX, B, N, w.xmax.as<float>());
+    EMSA_HIP_TRY(hipGetLastError());
+    EMSA_HIP_TRY(hipMemcpyAsync(&out, w.xmax.ptr, sizeof(float), hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));
+    return EMSA_OK;
+}
+
+template <int MODE>
+void sau_rows(bool ex, dim3 g, dim3 b, hipStream_t s, const Sau& d, int k, int it, float k1,
+              float k0) {
+    if (ex) hipLaunchKernelGGL((sau_rowpass_kernel<MODE, true>), g, b, 0, s, d, k, it, k1, k0);
+    else hipLaunchKernelGGL((sau_rowpass_kernel<MODE, false>), g, b, 0, s, d, k, it, k1, k0);
+}
+
+template <int NW>
+void sau_cols(bool ex, dim3 g, dim3 b, hipStream_t s, const Sau& d, const float* z, size_t zb,
+              float k1, float k0) {
+    if (ex) hipLaunchKernelGGL((sau_colpass_kernel<NW, true>), g, b, 0, s, d, z, zb, k1, k0);
+    else hipLaunchKernelGGL((sau_colpass_kernel<NW, false>), g, b, 0, s, d, z, zb, k1, k0);
+}
 
 SauWork* sau_work(const DeviceCtx& ctx) {
     if (ctx.device < 0 || ctx.device >= kMaxDevices) return nullptr;
@@ -1240,17 +1345,19 @@ int emsa_softassign_unrolled_fwd(const float* dX, const float* dY, int B, int N,
     const dim3 cgrid((unsigned)(d.colBlocks * d.H), ub), fgrid((unsigned)d.colBlocks, ub);
     const dim3 tgrid((unsigned)(d.ldk / 256), ub);
     const std::vector<float> Ts = sau_temps(a);
+    float xmax2 = 0.f;
+    if ((rc = sau_xmax2(*wp, dX, B, N, s, xmax2)) != EMSA_OK) return rc;
     for (int it = 0; it < d.nIt; ++it) {
         float k1, k0;
         sau_consts(Ts[it], alpha, k1, k0);
+        const bool ex = k1 * xmax2 <= kExpandMax;  // the exponent's form
         hipLaunchKernelGGL(sau_transform_kernel, tgrid, dim3(256), 0, s, d, it);
         for (int k = 1; k <= I1; ++k) {
-            hipLaunchKernelGGL(sau_rowpass_kernel<kRowFwd>, rgrid, blk, 0, s, d, k, it, k1, k0);
-            hipLaunchKernelGGL(sau_colpass_kernel<1>, cgrid, blk, 0, s, d, (const float*)d.u,
-                               (size_t)N, k1, k0);
+            sau_rows<kRowFwd>(ex, rgrid, blk, s, d, k, it, k1, k0);
+            sau_cols<1>(ex, cgrid, blk, s, d, (const float*)d.u, (size_t)N, k1, k0);
             hipLaunchKernelGGL(sau_colfin_kernel<kColFwd>, fgrid, blk, 0, s, d, k, it);
         }
-        hipLaunchKernelGGL(sau_rowpass_kernel<kRowFinal>, rgrid, blk, 0, s, d, I1, it, k1, k0);
+        sau_rows<kRowFinal>(ex, rgrid, blk, s, d, I1, it, k1, k0);
         hipLaunchKernelGGL(sau_solve_kernel, dim3(ub), blk, 0, s, d, it, Ts[it]);
         EMSA_HIP_TRY(hipGetLastError());
     }
@@ -1328,22 +1435,23 @@ int emsa_softassign_unrolled_bwd(const float* dX, const float* dY, int B, int N,
     const dim3 pgrid((unsigned)(d.rowTiles * d.fchunks), ub);
     const dim3 frgrid((unsigned)d.rowTiles, ub), fcgrid((unsigned)d.colBlocks, ub);
     const std::vector<float> Ts = sau_temps(a);
+    float xmax2 = 0.f;
+    if ((rc = sau_xmax2(*wp, dX, B, N, s, xmax2)) != EMSA_OK) return rc;
     for (int it = d.nIt - 1; it >= 0; --it) {
         const float T = Ts[it];
         float k1, k0;
         sau_consts(T, alpha, k1, k0);
+        const bool ex = k1 * xmax2 <= kExpandMax;  // the forward's form at this temperature
         const double fT = std::pow((double)TFACTOR, (double)(it / I0));
         hipLaunchKernelGGL(sau_transform_kernel, tgrid, dim3(256), 0, s, d, it);
         hipLaunchKernelGGL(sau_adj_solve_kernel, dim3(ub), dim3(64), 0, s, d, it);
         hipLaunchKernelGGL(sau_adj_mrow_kernel, mgrid, dim3(256), 0, s, d, it);
-        hipLaunchKernelGGL(sau_colpass_kernel<4>, cgrid, blk, 0, s, d, (const float*)d.zr,
-                           4 * Nz, k1, k0);
+        sau_cols<4>(ex, cgrid, blk, s, d, (const float*)d.zr, 4 * Nz, k1, k0);
         hipLaunchKernelGGL(sau_colfin_kernel<kColBwdM>, fgrid, blk, 0, s, d, I1, it);
         for (int k = I1; k >= 1; --k) {
-            hipLaunchKernelGGL(sau_rowpass_kernel<kRowBwd>, rgrid, blk, 0, s, d, k, it, k1, k0);
+            sau_rows<kRowBwd>(ex, rgrid, blk, s, d, k, it, k1, k0);
             if (k > 1) {
-                hipLaunchKernelGGL(sau_colpass_kernel<1>, cgrid, blk, 0, s, d, (const float*)d.zr,
-                                   4 * Nz, k1, k0);
+                sau_cols<1>(ex, cgrid, blk, s, d, (const float*)d.zr, 4 * Nz, k1, k0);
                 hipLaunchKernelGGL(sau_colfin_kernel<kColBwd>, fgrid, blk, 0, s, d, k - 1, it);
             }
         }
