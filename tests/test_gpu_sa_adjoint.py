@@ -94,6 +94,19 @@ def test_sa_adjoint_four_sweeps(UR):
     _check(UR, X, Y, Rt, JMAX=8, I1=4)
 
 
+def test_sa_adjoint_cold_schedule(UR):
+    """T from 0.02 down to 0.0063: k1 max|x|^2 (k1 = log2(e) / T, max|x|^2 =
+    0.49) crosses 64 at T = 0.011, so the first 6 temperatures run the
+    expanded exponent a_r + b_c + x'_r . p_c and the last 6 the direct
+    difference, in the forward and in the adjoint. A float32 run of the FP64
+    loop stays within 2e-5 of its gradients on this case."""
+    X, Y, Rt = _pair(256, 256, seed=3)
+    Ts = 0.02 * 0.9 ** np.arange(12)
+    k1x = 1.4426950408889634 / Ts * (X.astype(np.float64) ** 2).sum(1).max()
+    assert (k1x <= 64).any() and (k1x > 64).any()
+    _check(UR, X, Y, Rt, T_0=0.02, TFACTOR=0.9, JMAX=12)
+
+
 def test_sa_forward_matches_plain_softassign(UR):
     """The unrolled forward runs the iteration of the plain Softassign entry
     point (materialised variant), same float temperature schedule."""
