@@ -1138,24 +1138,27 @@ sau_adj_output_kernel(Sau d, float* __restrict__ gRt0, double* __restrict__ gT0,
 // ---------------------------------------------------------------------------
 // max over the batch's target clouds of |x|^2 into out[0] (one workgroup;
 // the values are non-negative, so their bit patterns order as the floats)
-__global__ void __launch_bounds__(1024) sau_xmax_kernel(const float* __restrict__ X, int B,
-                                                        int N, float* __restrict__ out) {
+// max |x|^2 over one chunk of kXmaxChunk points (blockIdx.x) of target cloud
+// blockIdx.y, written to out[blockIdx.y * gridDim.x + blockIdx.x]
+constexpr int kXmaxThreads = 256;
+constexpr int kXmaxChunk = 4096;
+__global__ void __launch_bounds__(kXmaxThreads) sau_xmax_kernel(const float* __restrict__ X,
+                                                                int N, float* __restrict__ out) {
+    const float* __restrict__ x = X + (size_t)blockIdx.y * 3 * N;
+    const int beg = blockIdx.x * kXmaxChunk, end = min(beg + kXmaxChunk, N);
     float m = 0.f;
-    const long long total = (long long)B * N;
-    for (long long i = threadIdx.x; i < total; i += blockDim.x) {
-        const long long b = i / N, r = i - b * N;
-        const float* x = X + (size_t)b * 3 * N;
+    for (int r = beg + (int)threadIdx.x; r < end; r += kXmaxThreads) {
         const float x0 = x[r], x1 = x[(size_t)N + r], x2 = x[2 * (size_t)N + r];
         m = fmaxf(m, fmaf(x0, x0, fmaf(x1, x1, x2 * x2)));
     }
     m = wave_max(m);
-    __shared__ float wm[1024 / kWave];
+    __shared__ float wm[kXmaxThreads / kWave];
     if ((threadIdx.x & (kWave - 1)) == 0) wm[threadIdx.x / kWave] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         float t = wm[0];
-        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) t = fmaxf(t, wm[w]);
-        out[0] = t;
+        for (int w = 1; w < kXmaxThreads / kWave; ++w) t = fmaxf(t, wm[w]);
+        out[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
     }
 }
 
@@ -1171,11 +1174,18 @@ SauWork* g_sau[kMaxDevices];  // per device, created under that device's lock, n
 // the exponent's form per temperature)
 int sau_xmax2(SauWork& w, const float* dX, int B, int N, hipStream_t s, float& out) {
     int rc;
-    if ((rc = w.xmax.reserve(sizeof(float) * 64)) != EMSA_OK) return rc;
-    hipLaunchKernelGGL(sau_xmax_kernel, dim3(1), dim3(1024), 0, s, dX, B, N, w.xmax.as<float>());
+    const int chunks = (N + kXmaxChunk - 1) / kXmaxChunk;
+    const size_t parts = (size_t)B * chunks;
+    if ((rc = w.xmax.reserve(sizeof(float) * parts)) != EMSA_OK) return rc;
+    hipLaunchKernelGGL(sau_xmax_kernel, dim3((unsigned)chunks, (unsigned)B), dim3(kXmaxThreads), 0,
+                       s, dX, N, w.xmax.as<float>());
     EMSA_HIP_TRY(hipGetLastError());
-    EMSA_HIP_TRY(hipMemcpyAsync(&out, w.xmax.ptr, sizeof(float), hipMemcpyDeviceToHost, s));
+    std::vector<float> h(parts);
+    EMSA_HIP_TRY(hipMemcpyAsync(h.data(), w.xmax.ptr, sizeof(float) * parts,
+                                hipMemcpyDeviceToHost, s));
     EMSA_HIP_TRY(hipStreamSynchronize(s));
+    out = 0.f;
+    for (float v : h) out = std::fmax(out, v);
     return EMSA_OK;
 }
 
