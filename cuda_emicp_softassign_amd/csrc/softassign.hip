@@ -515,6 +515,9 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
     }
     const float k2 = 2.f * k1;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    // 4 rows per trip: their scalar loads (x_r, f_r) are issued together
+    // instead of one wait per row (log domain at 5032^2: 2% per registration)
+#pragma unroll 4
     for (int r = rs; r < re; ++r) {
         if (LOG && EXP) {
             const float x0 = b.X[r], x1 = b.X[(size_t)N + r], x2 = b.X[2 * (size_t)N + r];
