@@ -22,10 +22,11 @@
 //    exp2(log2 K_rc + g_c) from the points, with f_r = log2 u_r and
 //    g_c = log2 v_c kept as potentials (one exp2 per element per pass, no
 //    HBM traffic for the matrix).
-// Launches per inner iteration (I1 = 3: 11), with no signalling between
-// workgroups: row passes give each wave whole rows (its sums need no other
-// workgroup); a column pass is a partial-sum kernel over row chunks plus a
-// finalize kernel per 64 columns; the final row pass writes per-workgroup
+// Launches per inner iteration (I1 = 3: 11 log domain, 8 materialised), with
+// no signalling between workgroups: row passes give each wave whole rows (its
+// sums need no other workgroup); a column pass writes partial sums (over row
+// chunks, or per workgroup of the one-pass materialised sweep) that a finalize
+// kernel per 64 columns adds up; the final row pass writes per-workgroup
 // FP64 moments that one solve workgroup reduces (fixed order) before Horn's
 // step, and the next inner iteration's first row pass transforms the sources
 // with the new (R, t) as it streams them.
@@ -67,8 +68,6 @@ struct SaPlan {
     int H = 0;          // row chunks per column block
     int chunkRows = 0;  // rows per chunk (multiple of kSaWaves)
     int fuseBlocks = 0; // ceil(N / kSaFuseRows) workgroups of the one-pass sweep
-    int pH = 0;         // chunks of the fuseBlocks partial rows of the one-pass sweep
-    int pChunkRows = 0; // partial rows per chunk (multiple of kSaWaves)
 };
 
 // Splits n rows into chunks so that colBlocks x chunks is about 4 workgroups
@@ -88,7 +87,6 @@ SaPlan plan_sa(int N, int M, int numCU) {
     p.colBlocks = p.ldk / kSaColsPB;
     split_rows(N, p.colBlocks, numCU, p.H, p.chunkRows);
     p.fuseBlocks = (N + kSaFuseRows - 1) / kSaFuseRows;
-    split_rows(p.fuseBlocks, p.colBlocks, numCU, p.pH, p.pChunkRows);
     return p;
 }
 
@@ -104,7 +102,7 @@ struct SaBufs {
     float* ocol;      // [M]
     float* bcol;      // [ldk] last column sums b_c
     float* wfin;      // [4][ldk] final-pass column weights (scale, scale b y)
-    float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [fuseBlocks + pH][ldk])
+    float* cpart;     // [H][ldk] column partial sums (one-pass sweep: [fuseBlocks][ldk])
     double* rmom;     // [kSaMomPad][final-pass workgroups] moments of the final pass
     double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
@@ -349,8 +347,8 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
 // sums with the current column scale give the new row scales (as the row
 // pass does), then the column partial sums over the workgroup's rows with the
 // new scales go to cpart[blockIdx.x] (as the column pass does over its row
-// chunk); sa_partsum_kernel reduces these fuseBlocks partial rows to pH
-// chunk sums, which the column finalize adds up. K is read once per sweep
+// chunk); the column finalize adds these fuseBlocks partial rows up. K is
+// read once per sweep
 // instead of twice. FIRST: the first sweep of an inner
 // iteration, which forms K from the transformed sources, stores it, and
 // starts from a fresh column scale v = 1.
@@ -561,69 +559,41 @@ sa_cols_kernel(SaBufs b, int N, int ldk, int H, int chunkRows, float k1, float k
     b.cpart[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
 }
 
-// One-pass sweep, column sums stage 2: the fuseBlocks partial rows of the
-// fused pass (in) reduced to pH partial rows (out) as stage 1 reduces K rows:
-// a workgroup = 256 columns x one chunk of partial rows, the 4 waves split the
-// chunk and combine in LDS in a fixed order.
-__global__ void __launch_bounds__(kWave * kSaWaves)
-sa_partsum_kernel(const float* __restrict__ in, float* __restrict__ out, int P, int ldk, int H,
-                  int chunkRows) {
-    const int cbk = blockIdx.x / H;
-    const int h = blockIdx.x - cbk * H;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int c = cbk * kSaColsPB + 4 * lane;
-    const int q = chunkRows / kSaWaves;
-    const int rs = h * chunkRows + wave * q;
-    const int re = min(rs + q, P);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int r = rs; r < re; ++r) {
-        float t[4];
-        load4(in + (size_t)r * ldk + c, t);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] += t[i];
-    }
-    __shared__ float red[kSaWaves][kSaColsPB];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) red[wave][4 * lane + i] = acc[i];
-    __syncthreads();
-    const int tid = threadIdx.x;
-    float s = red[0][tid];
-#pragma unroll
-    for (int w = 1; w < kSaWaves; ++w) s += red[w][tid];
-    out[(size_t)h * ldk + cbk * kSaColsPB + tid] = s;
-}
-
-// Column pass, stage 2: a workgroup finalizes 64 columns; its 4 waves sum
-// interleaved row chunks in FP64 and combine in LDS (fixed order). Then
+// Column pass, stage 2: a workgroup finalizes 64 columns; its NWF waves sum
+// interleaved partial rows in FP64 (U partial sums per lane, so U loads are
+// in flight, added pairwise) and combine in LDS (fixed order). Then
 // b_c = (column scale) * sum + o_col[c]; scale /= b_c; o_col /= b_c; b_c kept;
 // sum_c b_c y_c of the block for Yc. After the last sweep (last != 0) also the
-// column weights of the final row pass.
+// column weights of the final row pass. Log domain: the H row-chunk partials
+// of the column pass (4 waves, U = 4); materialised one-pass sweep: the
+// partial row of every fused-pass workgroup (N / 8 rows: 16 waves, U = 8).
 constexpr int kSaFinCols = kWave;
+constexpr int kSaFinWavesWide = 16;
 
-template <bool LOG, bool FIRST>
-__global__ void __launch_bounds__(kWave * kSaWaves)
+template <bool LOG, bool FIRST, int NWF = kSaWaves, int U = 4>
+__global__ void __launch_bounds__(kWave * NWF)
 sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
+    static_assert(U == 4 || U == 8, "pairwise combine of 4 or 8 partial sums");
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int cc = blockIdx.x * kSaFinCols + lane;
-    // four partial sums over interleaved chunks (four loads in flight per
-    // lane), added in a fixed order
-    double pa[4] = {0.0, 0.0, 0.0, 0.0};
-    int k = wave;
-    for (; k + 3 * kSaWaves < H; k += 4 * kSaWaves)
+    double pa[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pa[u] += b.cpart[(size_t)(k + u * kSaWaves) * ldk + cc];
-    for (; k < H; k += kSaWaves) pa[0] += b.cpart[(size_t)k * ldk + cc];
-    const double part = (pa[0] + pa[1]) + (pa[2] + pa[3]);
-    __shared__ double red[kSaWaves][kSaFinCols];
+    for (int u = 0; u < U; ++u) pa[u] = 0.0;
+    int k = wave;
+    for (; k + (U - 1) * NWF < H; k += U * NWF)
+#pragma unroll
+        for (int u = 0; u < U; ++u) pa[u] += b.cpart[(size_t)(k + u * NWF) * ldk + cc];
+    for (; k < H; k += NWF) pa[0] += b.cpart[(size_t)k * ldk + cc];
+    double part = (pa[0] + pa[1]) + (pa[2] + pa[3]);
+    if (U == 8) part += (pa[4 % U] + pa[5 % U]) + (pa[6 % U] + pa[7 % U]);
+    __shared__ double red[NWF][kSaFinCols];
     red[wave][lane] = part;
     __syncthreads();
     if (wave != 0) return;
     double tot = red[0][lane];
 #pragma unroll
-    for (int w = 1; w < kSaWaves; ++w) tot += red[w][lane];
+    for (int w = 1; w < NWF; ++w) tot += red[w][lane];
     double by[3] = {0.0, 0.0, 0.0};
     float w0 = 0.f, wy[3] = {0.f, 0.f, 0.f};
     if (cc < M) {
@@ -841,7 +811,7 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     constexpr int slabCols = kSaColsPB * kSaFuseWaves;
     const int slabs =
         (!LOGV && pl.ldk <= kSaFuseMaxSlabs * slabCols) ? (pl.ldk + slabCols - 1) / slabCols : 0;
-    const size_t partRows = slabs > 0 ? (size_t)pl.fuseBlocks + pl.pH : (size_t)pl.H;
+    const size_t partRows = slabs > 0 ? (size_t)pl.fuseBlocks : (size_t)pl.H;
     if ((rc = ctx.colpart.reserve(sizeof(float) * partRows * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
@@ -852,18 +822,12 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
-    // one-pass sweep: the fused pass writes fuseBlocks partial rows, stage 2
-    // reduces them to pH rows past them, which the finalize reads (bf: b with
-    // every field set, cpart moved)
-    SaBufs bf = b;
-    bf.cpart = b.cpart + (size_t)pl.fuseBlocks * pl.ldk;
-
     hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
                        p.moutlier);
     EMSA_HIP_TRY(hipGetLastError());
     const dim3 rgrid((unsigned)pl.rowBlocks), cgrid((unsigned)(pl.colBlocks * pl.H)),
-        fgrid((unsigned)finBlocks), pgrid((unsigned)(pl.colBlocks * pl.pH));
-    const dim3 blk(kWave * kSaWaves);
+        fgrid((unsigned)finBlocks);
+    const dim3 blk(kWave * kSaWaves), wblk(kWave * kSaFinWavesWide);
     const dim3 xgrid((unsigned)pl.fuseBlocks), xblk(kWave * kSaFuseWaves);
     float T = p.T_0;
     int it = 0;
@@ -897,14 +861,12 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                     if ((rc = launch_rowcol(sw == 0, slabs, xgrid, xblk, s, b, N, M, pl.ldk, k1,
                                             k0)) != EMSA_OK)
                         return rc;
-                    hipLaunchKernelGGL(sa_partsum_kernel, pgrid, blk, 0, s, b.cpart, bf.cpart,
-                                       pl.fuseBlocks, pl.ldk, pl.pH, pl.pChunkRows);
                     if (sw == 0)
-                        hipLaunchKernelGGL((sa_colfin_kernel<false, true>), fgrid, blk, 0, s, bf,
-                                           M, pl.ldk, pl.pH, last);
+                        hipLaunchKernelGGL((sa_colfin_kernel<false, true, kSaFinWavesWide, 8>),
+                                           fgrid, wblk, 0, s, b, M, pl.ldk, pl.fuseBlocks, last);
                     else
-                        hipLaunchKernelGGL((sa_colfin_kernel<false, false>), fgrid, blk, 0, s, bf,
-                                           M, pl.ldk, pl.pH, last);
+                        hipLaunchKernelGGL((sa_colfin_kernel<false, false, kSaFinWavesWide, 8>),
+                                           fgrid, wblk, 0, s, b, M, pl.ldk, pl.fuseBlocks, last);
                 } else {
                     if (sw == 0)
                         hipLaunchKernelGGL((sa_rows_kernel<false, 0>), rgrid, blk, 0, s, b, N, M,
