@@ -160,6 +160,52 @@ def emicp_step(cloud_target, cloud_source, R, t, sigma_p2, d_02=0.01):
     return dict(C=Cr, lam=lam, record=rec, R=Rf.reshape(3, 3), t=tf)
 
 
+def icp(cloud_target, cloud_source, h_R, h_t, param=None):
+    """ICP with exact nearest-neighbour correspondences on the GPU
+    (src/icp.cpp:122-289, param.maxIteration iterations). Returns (R, t) and
+    also writes them into h_R, h_t when those are numpy arrays."""
+    param = param or registrationParameters()
+    X, N = cloud2data(cloud_target)
+    Y, M = cloud2data(cloud_source)
+    R, t = _inout(h_R, h_t)
+    p = param.to_c()
+    _lib.check("icp", _lib.load().emsa_icp(_fptr(X), N, _fptr(Y), M, _fptr(R), _fptr(t),
+                                           C.byref(p)))
+    return _writeback(h_R, h_t, R, t)
+
+
+def icp_traced(cloud_target, cloud_source, h_R, h_t, param=None):
+    """As icp, also returning the per-iteration trace (n, 32): iteration, mean
+    squared correspondence distance, Xc(3), Yc(3), S(9), R(9), t(3), -, -,
+    status."""
+    param = param or registrationParameters()
+    X, N = cloud2data(cloud_target)
+    Y, M = cloud2data(cloud_source)
+    R, t = _inout(h_R, h_t)
+    p = param.to_c()
+    n = max(int(param.maxIteration), 1)
+    tr = np.zeros((n, TRACE_W))
+    rc = _lib.check("icp_traced", _lib.load().emsa_icp_traced(
+        _fptr(X), N, _fptr(Y), M, _fptr(R), _fptr(t), C.byref(p), _dptr(tr), n))
+    Rr, tt = _writeback(h_R, h_t, R, t)
+    return Rr, tt, tr[:rc]
+
+
+def icp_nearest(cloud_target, cloud_source, R, t):
+    """The correspondences one ICP iteration uses: for each source point y_r,
+    the index of the target point nearest to R y_r + t and the squared
+    distance (FP32)."""
+    X, N = cloud2data(cloud_target)
+    Y, M = cloud2data(cloud_source)
+    Rf, tf = _inout(R, t)
+    idx = np.zeros(M, np.int32)
+    d2 = np.zeros(M, np.float32)
+    _lib.check("icp_nearest", _lib.load().emsa_icp_nearest(
+        _fptr(X), N, _fptr(Y), M, _fptr(Rf), _fptr(tf),
+        idx.ctypes.data_as(C.POINTER(C.c_int)), _fptr(d2)))
+    return idx, d2
+
+
 def _variant(v):
     if isinstance(v, str):
         return {"materialised": SOFTASSIGN_MATERIALISED, "logdomain": SOFTASSIGN_LOGDOMAIN,

@@ -50,6 +50,7 @@ class emsa_pair_grad(C.Structure):
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _pp = C.POINTER(emsa_params)
+_ip = C.POINTER(C.c_int)
 SIGNATURES = {
     "emsa_default_params": (None, [_pp]),
     "emsa_last_error": (C.c_char_p, []),
@@ -62,6 +63,9 @@ SIGNATURES = {
     "emsa_softassign_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int,
                                          _dp, C.c_int]),
     "emsa_find_rt_from_s": (C.c_int, [_fp, _fp, _fp, _fp, _fp]),
+    "emsa_icp": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp]),
+    "emsa_icp_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, _dp, C.c_int]),
+    "emsa_icp_nearest": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _ip, _fp]),
     "emsa_emicp_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, _pp,
                                  C.c_void_p]),
     "emsa_probe_rates": (C.c_int, [_dp, _dp]),

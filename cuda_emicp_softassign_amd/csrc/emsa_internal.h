@@ -132,6 +132,9 @@ struct DeviceCtx {
     EmWork em;
     // softassign workspaces
     DevBuf K, u, v, orow, ocol, colpart, samom, sastate;
+    // ICP workspaces: per-tile nearest candidates, row-block moments, state,
+    // per-row neighbour and distance (the correspondence query)
+    DevBuf icpDist, icpIdx, icpMom, icpState, icpRowIdx, icpRowDist;
     // captured EM-ICP launch sequences of the device entry points, replayed
     // when a call repeats the same buffers, sizes and parameters; keys seen
     // once (a sequence is recorded on its second call)

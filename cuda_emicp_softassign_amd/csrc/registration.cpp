@@ -61,6 +61,15 @@ void emicp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
                                h_t, &p));
 }
 
+void icp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
+         const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
+         const registrationParameters& param) {
+    const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
+    const emsa_params p = to_params(param);
+    report("icp", emsa_icp(X.data(), (int)(X.size() / 3), Y.data(), (int)(Y.size() / 3), h_R, h_t,
+                           &p));
+}
+
 void softassign(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
                 const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
                 const registrationParameters& param) {

@@ -101,6 +101,11 @@ void softassign(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
 void emicp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
            const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
            const registrationParameters& param);
+// ICP with exact nearest-neighbour correspondences (src/icp.cpp:122-289,
+// declared at src/3dregistration.h:96-99): param.maxIteration iterations.
+void icp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
+         const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
+         const registrationParameters& param);
 
 void findRTfromS(const float* h_Xc, const float* h_Yc, const float* h_S, float* h_R, float* h_t);
 

@@ -53,7 +53,7 @@ typedef struct emsa_params {
     float sigma_inf;  /* EM-ICP final sigma^2, default 1e-5 */
     float sigma_factor; /* EM-ICP sigma^2 factor, default 0.9 */
     float d_02;       /* EM-ICP outlier distance^2, default 0.01 */
-    int maxIteration; /* ICP only (not provided by this library), default 30 */
+    int maxIteration; /* ICP iterations (emsa_icp), default 30 */
     int noviewer;
     int nostop;
     int notimer;
@@ -106,6 +106,31 @@ int emsa_softassign_traced(const float *X, int N, const float *Y, int M, float *
  * src/3dregistration.h:104-107). S is h_S of the reference: S[3a+b] =
  * sum (source)_a (target)_b. */
 int emsa_find_rt_from_s(const float *Xc, const float *Yc, const float *S, float *R, float *t);
+
+/* ICP with hard correspondences on the GPU. Replaces icp() (src/icp.cpp:122-289;
+ * declared at src/3dregistration.h:96-99): p->maxIteration iterations of
+ * nearest source-to-target correspondences, centroids, S and Horn's step.
+ * The reference's FLANN kd-tree search (src/icp.cpp:175, :218-222) is
+ * approximate; here every target point is compared (exact nearest neighbour
+ * of the FP32 distance with the transformed source point compensated for its
+ * FP32 rounding; equal distances resolve to the lowest target index).
+ * R, t in/out as in emsa_emicp; maxIteration <= 0 leaves them unchanged. */
+int emsa_icp(const float *X, int N, const float *Y, int M, float *R, float *t,
+             const emsa_params *p);
+
+/* As emsa_icp with one EMSA_TRACE_W record per iteration: iteration, mean
+ * squared correspondence distance, Xc(3), Yc(3), h_S(9), R(9), t(3), -, -,
+ * status. Returns the number of records written or a negative code. */
+int emsa_icp_traced(const float *X, int N, const float *Y, int M, float *R, float *t,
+                    const emsa_params *p, double *trace, int trace_cap);
+
+/* One correspondence query of emsa_icp (no counterpart function in the
+ * reference; it is the knnSearch of src/icp.cpp:218-222 with k = 1): for
+ * every source point y_r, idx[r] receives the target point nearest to
+ * R y_r + t and dist2[r] its squared distance (FP32), as emsa_icp finds them.
+ * R, t are read only. */
+int emsa_icp_nearest(const float *X, int N, const float *Y, int M, const float *R,
+                     const float *t, int *idx, float *dist2);
 
 /* ---- Device-buffer entry points (inputs resident in HBM) ---------------- */
 

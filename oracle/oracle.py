@@ -184,3 +184,50 @@ def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col, precision="f64"):
         soa(X, dt), N, soa(Y, dt), M, R, t, T, alpha, I1, o_row, o_col, Mat, rs, cs, Xc, Yc, S)
     return dict(rowsum=rs, colsum=cs, Xc=Xc, Yc=Yc, S=S, R=R.reshape(3, 3), t=t, sumM=sumM,
                 Mat=Mat.reshape(N, M))
+
+
+def icp_nearest(X, P):
+    """Exact nearest neighbour in FP64: for each row of P, the index of the
+    nearest row of X (squared Euclidean distance; the lowest index among
+    equal distances) and that distance. The reference searches with FLANN's
+    kd-tree (src/icp.cpp:175, :218-222, k = 1); this is the exact search it
+    approximates."""
+    X = np.asarray(X, np.float64)
+    P = np.asarray(P, np.float64)
+    idx = np.zeros(len(P), np.int64)
+    d2 = np.zeros(len(P))
+    chunk = max(1, 2_000_000 // max(len(X), 1))
+    for s in range(0, len(P), chunk):
+        Q = P[s:s + chunk]
+        # |x - p|^2 per pair, formed from the differences (no expansion)
+        D = ((X[None, :, :] - Q[:, None, :]) ** 2).sum(2)
+        k = np.argmin(D, axis=1)
+        idx[s:s + chunk] = k
+        d2[s:s + chunk] = ((X[k] - Q) ** 2).sum(1)
+    return idx, d2
+
+
+def icp(X, Y, R0=None, t0=None, maxIteration=30):
+    """ICP in FP64 with exact nearest neighbours (src/icp.cpp:122-289):
+    Yc = mean Y once (:162); per iteration p = R y + t (:208-210), the nearest
+    target point of each p (:218-229), Xc = mean of those (:234),
+    S[i + 3j] = sum (x_n - Xc)_i (y - Yc)_j (:254-258), then findRTfromS (:268)
+    through the C oracle's Horn step. Returns (R, t, trace) with trace rows
+    [iteration, mean squared distance, Xc(3), Yc(3), h_S(9), R(9), t(3), 0, 0, 0]."""
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    R = np.eye(3) if R0 is None else np.asarray(R0, np.float64).reshape(3, 3)
+    t = np.zeros(3) if t0 is None else np.asarray(t0, np.float64).reshape(3)
+    Yc = Y.mean(0)
+    trace = []
+    for it in range(maxIteration):
+        idx, d2 = icp_nearest(X, Y @ R.T + t)
+        Xn = X[idx]
+        Xc = Xn.mean(0)
+        C = (Xn - Xc).T @ (Y - Yc)          # C[i, j] = sum (x - Xc)_i (y - Yc)_j
+        S = C.T.reshape(9)                  # S[i + 3j] = C[i, j]
+        Rn, tn, rc = find_rt_from_s(Xc, Yc, S)
+        if rc >= 0 and np.all(np.isfinite(Rn)):
+            R, t = Rn, tn
+        trace.append(np.concatenate([[it, d2.mean()], Xc, Yc, S, R.reshape(9), t, [0, 0, 0]]))
+    return R, t, np.array(trace).reshape(-1, TRACE_W)

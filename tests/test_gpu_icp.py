@@ -1,0 +1,132 @@
+"""ICP on the GPU (csrc/icp.hip; src/icp.cpp:122-289) against the FP64 oracle
+(oracle.icp, oracle.icp_nearest, pinned in tests/test_oracle_icp.py).
+
+Correspondences: every source point's neighbour equals the exact FP64 nearest
+neighbour wherever the FP64 gap between the nearest and the second nearest
+squared distance exceeds 1e-5 relative (the FP32 distance cannot separate
+closer candidates), and otherwise lies within 1e-5 relative of the nearest;
+distances within 1e-5 relative (+1e-12). Registrations: R within 1e-5 rad and
+t within 1e-5 of the FP64 loop, every iteration's Xc and Yc within 1e-5.
+"""
+import numpy as np
+import pytest
+
+from cuda_emicp_softassign_amd.clouds import geodesic, random_rotation, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def E():
+    import cuda_emicp_softassign_amd as E
+    return E
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O
+    return O
+
+
+def _pair(n, m, seed, deg=10.0, tmax=0.03, noise=1e-3):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-0.5, 0.5, size=(n, 3))
+    R = random_rotation(rng, deg)
+    t = rng.uniform(-tmax, tmax, 3)
+    sel = rng.permutation(n)[:m] if m <= n else rng.integers(0, n, m)
+    Y = (X[sel] - t) @ R + rng.normal(0, noise, (m, 3))
+    return X.astype(np.float32), Y.astype(np.float32), R, t
+
+
+def _check_nearest(E, O, X, Y, R, t):
+    print("nearest N=%d M=%d" % (len(X), len(Y)), flush=True)
+    idx, d2 = E.icp_nearest(X, Y, R, t)
+    # the library receives R, t in FP32 and transforms in FP64
+    R32 = np.asarray(R, np.float32).astype(np.float64)
+    t32 = np.asarray(t, np.float32).astype(np.float64)
+    P = Y.astype(np.float64) @ R32.T + t32
+    X64 = X.astype(np.float64)
+    ko, do = O.icp_nearest(X64, P)
+    dg = ((X64[idx] - P) ** 2).sum(1)
+    assert idx.min() >= 0 and idx.max() < len(X)
+    # second nearest distance (FP64) to find the rows a FP32 distance can separate
+    if len(X) > 1:
+        D = ((X64[None, :, :] - P[:, None, :]) ** 2).sum(2) if len(X) * len(P) <= 4_000_000 \
+            else None
+        if D is not None:
+            D[np.arange(len(P)), ko] = np.inf
+            second = D.min(1)
+            clear = second - do > 1e-5 * np.maximum(do, 1e-12)
+        else:
+            clear = np.ones(len(P), bool)
+    else:
+        clear = np.ones(len(P), bool)
+    assert np.array_equal(idx[clear], ko[clear]), np.flatnonzero(idx[clear] != ko[clear])[:5]
+    assert np.all(dg <= do * (1 + 1e-5) + 1e-12)
+    np.testing.assert_allclose(d2, dg, rtol=1e-5, atol=1e-12)
+    return clear.mean()
+
+
+def test_nearest_exact(E, O):
+    X, Y, R, t = _pair(1500, 1300, seed=1)
+    frac = _check_nearest(E, O, X, Y, R, t)
+    assert frac > 0.99
+
+
+@pytest.mark.parametrize("N,M", [(1, 5), (5, 1), (17, 3000), (3000, 17), (4097, 65),
+                                 (2049, 700), (300, 2049)])
+def test_nearest_ragged(E, O, N, M):
+    rng = np.random.default_rng(N * 7 + M)
+    X = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
+    Y = rng.uniform(-1, 1, (M, 3)).astype(np.float32)
+    R = random_rotation(rng, 40.0)
+    t = rng.uniform(-0.1, 0.1, 3)
+    _check_nearest(E, O, X, Y, R, t)
+
+
+def test_nearest_10k(E, O):
+    """Many column chunks per row block (N = M = 10,000)."""
+    X, Y, R, t = _pair(10000, 10000, seed=3)
+    _check_nearest(E, O, X, Y, R, t)
+
+
+@pytest.mark.parametrize("seed,n,m", [(5, 1007, 1007), (6, 2000, 1500), (7, 700, 1200)])
+def test_icp_matches_oracle(E, O, seed, n, m):
+    X, Y, Rg, tg = _pair(n, m, seed=seed)
+    print("icp N=%d M=%d seed=%d" % (n, m, seed), flush=True)
+    p = E.registrationParameters(maxIteration=30)
+    R, t, tr = E.icp_traced(X, Y, np.eye(3), np.zeros(3), p)
+    Ro, to, tro = O.icp(X, Y, maxIteration=30)
+    assert len(tr) == 30 and len(tro) == 30
+    assert geodesic(R.astype(np.float64), Ro) < 1e-5
+    np.testing.assert_allclose(t, to, atol=1e-5)
+    np.testing.assert_allclose(tr[:, 2:8], tro[:, 2:8], atol=1e-5)
+    np.testing.assert_allclose(tr[:, 1], tro[:, 1], rtol=1e-3, atol=1e-9)
+    assert not tr[:, 31].any()
+    # the result is the drop-in icp()'s too
+    R2, t2 = E.icp(X, Y, np.eye(3), np.zeros(3), p)
+    assert np.array_equal(R2, R) and np.array_equal(t2, t)
+
+
+def test_icp_recovers_noiseless_transform(E):
+    X, Y, Rg, tg = _pair(3000, 3000, seed=8, deg=8.0, tmax=0.02, noise=0.0)
+    R, t = E.icp(X, Y, np.eye(3), np.zeros(3), E.registrationParameters(maxIteration=40))
+    assert geodesic(R.astype(np.float64), Rg) < 1e-5
+    np.testing.assert_allclose(t, tg, atol=1e-5)
+
+
+def test_icp_zero_iterations_and_errors(E):
+    X, Y, _, _ = _pair(100, 90, seed=9)
+    hR = np.eye(3, dtype=np.float32)
+    ht = np.zeros(3, np.float32)
+    R, t = E.icp(X, Y, hR, ht, E.registrationParameters(maxIteration=0))
+    assert np.array_equal(R, np.eye(3)) and not t.any()
+    with pytest.raises(E.EmsaError) as ei:
+        E.icp(np.zeros((0, 3)), Y, hR, ht)
+    assert ei.value.code == E._lib.EMSA_EINVAL
+    Xn = X.copy()
+    Xn[4, 0] = np.nan
+    with pytest.raises(E.EmsaError) as ei:
+        E.icp(Xn, Y, hR, ht)
+    assert ei.value.code == E._lib.EMSA_ENONFINITE
+    assert np.array_equal(hR, np.eye(3)) and not ht.any()
