@@ -240,6 +240,35 @@ def config3(E, torch, dev, sptr, timed, steps, p, n_inst=256):
             "frac_within_1e-4_rad": float(np.mean(errs < 1e-4))}
 
 
+def icp_plain(E, O, n=10000, iters=30):
+    """ICP (hard correspondences, the reference's default 30 iterations) on a
+    pair of N = M = 10,000 uniform-cube clouds, one registration after a
+    warm-up. Each iteration compares every (source, target) pair once: N M
+    pairs of the exact nearest-neighbour search (compensated difference,
+    |d|^2, compare-and-select: about 11 VALU per pair). Agreement with the
+    FP64 oracle is checked over the first 3 iterations of the same pair (the
+    oracle's search takes seconds per iteration at this size)."""
+    from cuda_emicp_softassign_amd.clouds import geodesic, uniform_cube
+    X, Y, _, _ = uniform_cube(n, seed=21)
+    p = E.registrationParameters(maxIteration=iters)
+    E.icp(X, Y, np.eye(3), np.zeros(3), p)
+    t0 = time.perf_counter()
+    R, t = E.icp(X, Y, np.eye(3), np.zeros(3), p)
+    ms = (time.perf_counter() - t0) * 1e3
+    pair_rate = float(n) * n * iters / (ms * 1e-3)
+    out = {"workload": "ICP N=M=%d uniform cube, %d iterations, exact nearest neighbours"
+                       % (n, iters),
+           "ms_per_registration": ms, "pairs_per_s": pair_rate,
+           "registrations_per_s": 1e3 / ms}
+    if O is not None:
+        p3 = E.registrationParameters(maxIteration=3)
+        R3, t3 = E.icp(X, Y, np.eye(3), np.zeros(3), p3)
+        Ro, to, _ = O.icp(X, Y, maxIteration=3)
+        out["rot_err_vs_fp64_oracle_rad_3_iterations"] = geodesic(R3.astype(np.float64), Ro)
+        out["t_err_vs_fp64_oracle_3_iterations"] = float(np.abs(t3 - to).max())
+    return out
+
+
 def softassign_plain(E, n=5032):
     """Softassign at the reference defaults (JMAX 100, I0 5, I1 3, alpha 3,
     T_0 100, TFACTOR 0.95: 500 inner iterations), N = M = 5032 surface pairs
@@ -350,6 +379,7 @@ def main():
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the 256-instance x 24-start batched workload")
     ap.add_argument("--no-grad", action="store_true", help="skip the fwd+bwd workloads")
+    ap.add_argument("--no-icp", action="store_true", help="skip the N=M=10k ICP timing")
     ap.add_argument("--no-softassign", action="store_true",
                     help="skip the N=M=5032 Softassign timing")
     ap.add_argument("--cull-sweep", action="store_true",
@@ -559,6 +589,11 @@ def main():
         out["batched_n1007"] = small_batched(E, torch, dev, sptr, timed, args.steps, p)
     if not args.no_softassign:
         out["softassign"] = softassign_plain(E)
+    if not args.no_icp:
+        O = None
+        if not args.no_check:
+            from oracle import oracle as O
+        out["icp"] = icp_plain(E, O)
 
 
     if not args.no_cpu:
