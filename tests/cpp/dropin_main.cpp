@@ -1,5 +1,5 @@
 // A caller written against the reference API (src/3dregistration.h): builds
-// two clouds, runs emicp() and softassign() through the drop-in header and
+// two clouds, runs emicp(), softassign() and icp() through the drop-in header and
 // checks that the known rigid transform comes back. Exit status 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -71,6 +71,17 @@ int main() {
     printRT(s_R, s_t);
     for (int i = 0; i < 9; ++i)
         if (!std::isfinite(s_R[i])) return 4;
+
+    // icp(): noiseless pair, every source point has its exact partner
+    float i_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, i_t[3] = {0, 0, 0};
+    param.maxIteration = 60;
+    icp(target, source, i_R, i_t, param);
+    printRT(i_R, i_t);
+    double err3 = 0;
+    for (int i = 0; i < 9; ++i) err3 = std::fmax(err3, std::fabs(i_R[i] - Rt[i]));
+    for (int i = 0; i < 3; ++i) err3 = std::fmax(err3, std::fabs(i_t[i] - tt[i]));
+    std::printf("icp max |error| %g\n", err3);
+    if (!(err3 < 1e-4)) return 5;
     std::printf("dropin OK\n");
     return 0;
 }
