@@ -149,6 +149,8 @@ int emsa_emicp_dev(const float* dX, int N, const float* dY, int M, float* dRt,
     if (!dX || !dY || !dRt || !p || N <= 0 || M <= 0)
         return set_error("emicp_dev: bad arguments"), EMSA_EINVAL;
     if (!finite_params(*p)) return set_error("emicp_dev: non-finite params"), EMSA_ENONFINITE;
+    int rc = check_schedule(*p, "emicp_dev");
+    if (rc != EMSA_OK) return rc;
     const std::vector<float> sig = sigma_schedule(*p);
     if (sig.empty()) return EMSA_OK;
     DeviceCtx* ctx = current_ctx();
@@ -169,6 +171,7 @@ int emsa_emicp_batched_dev(const emsa_pair* pairs, int count, const float* dRt0,
     if (!dRt0 || !dRtOut || !p || n_starts <= 0)
         return set_error("emicp_batched_dev: bad arguments"), EMSA_EINVAL;
     if (!finite_params(*p)) return set_error("emicp_batched_dev: non-finite params"), EMSA_ENONFINITE;
+    if ((rc = check_schedule(*p, "emicp_batched_dev")) != EMSA_OK) return rc;
     const std::vector<float> sig = sigma_schedule(*p);
     DeviceCtx* ctx = current_ctx();
     if (!ctx) return EMSA_ENODEV;
@@ -201,6 +204,7 @@ int emsa_emicp_batched(const emsa_pair* pairs, int count, const float* Rt0, int 
     if (!Rt0 || !Rt_out || !p || n_starts <= 0)
         return set_error("emicp_batched: bad arguments"), EMSA_EINVAL;
     if (!finite_params(*p)) return set_error("emicp_batched: non-finite params"), EMSA_ENONFINITE;
+    if ((rc = check_schedule(*p, "emicp_batched")) != EMSA_OK) return rc;
     const size_t nregs = (size_t)count * n_starts;
     for (size_t i = 0; i < 12 * nregs; ++i)
         if (!std::isfinite(Rt0[i])) return set_error("emicp_batched: non-finite Rt0"), EMSA_ENONFINITE;
@@ -261,6 +265,7 @@ int emsa_autotune_emicp(const emsa_pair* pairs, int count, int n_starts, const e
     int rc;
     if ((rc = check_pairs(pairs, count)) != EMSA_OK) return rc;
     if (!p || n_starts <= 0) return set_error("autotune: bad arguments"), EMSA_EINVAL;
+    if ((rc = check_schedule(*p, "autotune")) != EMSA_OK) return rc;
     std::vector<float> sig = sigma_schedule(*p);
     if (sig.empty()) return set_error("autotune: the schedule runs no sigma^2 step"), EMSA_EINVAL;
     // 6 steps spread over the schedule: the large-sigma steps visit nearly

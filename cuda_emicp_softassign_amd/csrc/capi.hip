@@ -302,7 +302,10 @@ void emsa_default_params(emsa_params* p) {
 const char* emsa_last_error(void) { return g_err.c_str(); }
 
 int emsa_emicp_iterations(const emsa_params* p) {
-    if (!p) return EMSA_EINVAL;
+    clear_error();
+    if (!p) return set_error("emicp_iterations: null params"), EMSA_EINVAL;
+    const int rc = check_schedule(*p, "emicp_iterations");
+    if (rc != EMSA_OK) return rc;
     return (int)sigma_schedule(*p).size();
 }
 
@@ -310,6 +313,8 @@ int emsa_emicp(const float* X, int N, const float* Y, int M, float* R, float* t,
                const emsa_params* p) {
     clear_error();
     if (!p) return set_error("emicp: null params"), EMSA_EINVAL;
+    const int rc = check_schedule(*p, "emicp");
+    if (rc != EMSA_OK) return rc;
     return emicp_host(X, N, Y, M, R, t, *p, sigma_schedule(*p), nullptr, 0, nullptr, nullptr);
 }
 
@@ -317,6 +322,8 @@ int emsa_emicp_traced(const float* X, int N, const float* Y, int M, float* R, fl
                       const emsa_params* p, double* trace, int trace_cap) {
     clear_error();
     if (!p || !trace || trace_cap < 0) return set_error("emicp: bad trace args"), EMSA_EINVAL;
+    const int crc = check_schedule(*p, "emicp_traced");
+    if (crc != EMSA_OK) return crc;
     const std::vector<float> sig = sigma_schedule(*p);
     const int rc = emicp_host(X, N, Y, M, R, t, *p, sig, trace, trace_cap, nullptr, nullptr);
     return rc == EMSA_OK ? (int)std::min<size_t>(sig.size(), (size_t)trace_cap) : rc;

@@ -973,6 +973,7 @@ EmPlan plan_emicp(int N, int M, int numCU, int blocks_per_cu) {
 
 std::vector<float> sigma_schedule(const emsa_params& p) {
     std::vector<float> out;
+    if (check_schedule(p, "") != EMSA_OK) return out;
     float s = p.sigma_p2;
     while (s > p.sigma_inf) {  // emicp.cu:443
         out.push_back(s);
@@ -980,6 +981,15 @@ std::vector<float> sigma_schedule(const emsa_params& p) {
         if (out.size() > 1000000) break;
     }
     return out;
+}
+
+int check_schedule(const emsa_params& p, const char* fn) {
+    if (!(p.sigma_p2 > p.sigma_inf)) return EMSA_OK;  // the loop body never runs
+    if (p.sigma_factor > 0.f && p.sigma_factor < 1.f && p.sigma_inf >= 0.f) return EMSA_OK;
+    set_error(std::string(fn) +
+              ": sigma_factor must lie in (0, 1) and sigma_inf be >= 0 when sigma_p2 > "
+              "sigma_inf (the sigma^2 loop would not end)");
+    return EMSA_EINVAL;
 }
 
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,

@@ -165,6 +165,10 @@ DeviceCtx* current_ctx();
 
 // The sigma^2 schedule in float, exactly as the reference loop runs it.
 std::vector<float> sigma_schedule(const emsa_params& p);
+// EMSA_EINVAL (error set, prefixed by fn) when that loop would not end:
+// sigma_p2 > sigma_inf with sigma_factor outside (0, 1) or sigma_inf < 0 (the
+// reference loops forever there, src/emicp.cu:443, :668); else EMSA_OK.
+int check_schedule(const emsa_params& p, const char* fn);
 
 EmPlan plan_emicp(int N, int M, int numCU, int chunks_override = 0);
 

@@ -112,3 +112,23 @@ def test_dropin_header_compiles_without_pcl(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_endless_sigma_schedule_rejected(L):
+    """sigma_factor outside (0, 1) (or sigma_inf < 0) with sigma_p2 > sigma_inf:
+    the reference's loop (src/emicp.cu:443, :668) never ends; here every
+    EM-ICP entry point returns EMSA_EINVAL before touching a device."""
+    import ctypes
+    import cuda_emicp_softassign_amd as E
+    X = np.random.default_rng(1).uniform(size=(10, 3)).astype(np.float32)
+    for kw in (dict(sigma_factor=1.0), dict(sigma_factor=1.5), dict(sigma_factor=0.0),
+               dict(sigma_factor=-0.5), dict(sigma_inf=-1e-5)):
+        p = E.registrationParameters(**kw)
+        assert L.emsa_emicp_iterations(ctypes.byref(p.to_c())) == E._lib.EMSA_EINVAL
+        with pytest.raises(E.EmsaError) as ei:
+            E.emicp(X, X, np.eye(3), np.zeros(3), p)
+        assert ei.value.code == E._lib.EMSA_EINVAL
+        assert "sigma_factor" in str(ei.value)
+    # a schedule that runs no step is fine whatever the factor
+    p = E.registrationParameters(sigma_p2=1e-6, sigma_inf=1e-5, sigma_factor=2.0)
+    assert L.emsa_emicp_iterations(ctypes.byref(p.to_c())) == 0
