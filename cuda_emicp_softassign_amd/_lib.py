@@ -68,6 +68,8 @@ SIGNATURES = {
     "emsa_icp_nearest": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _ip, _fp]),
     "emsa_emicp_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, _pp,
                                  C.c_void_p]),
+    "emsa_emicp_dev_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                         _pp, C.c_void_p]),
     "emsa_probe_rates": (C.c_int, [_dp, _dp]),
     "emsa_emicp_batched": (C.c_int, [C.POINTER(emsa_pair), C.c_int, _fp, C.c_int, _pp, _fp,
                                      _dp]),

@@ -109,6 +109,12 @@ DeviceCtx* current_ctx() {
     return g_ctx[dev].get();
 }
 
+DeviceCtx* find_ctx(int dev) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (dev < 0 || dev >= (int)g_ctx.size()) return nullptr;
+    return g_ctx[dev].get();
+}
+
 int find_rt_device(DeviceCtx& ctx, const float* hXc, const float* hYc, const float* hS, float* hR,
                    float* ht) {
     int rc;

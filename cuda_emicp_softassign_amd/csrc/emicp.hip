@@ -992,21 +992,16 @@ int check_schedule(const emsa_params& p, const char* fn) {
     return EMSA_EINVAL;
 }
 
-int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
-               int tileCols, double* trace, double* rows, hipStream_t s, const EmAux* aux,
-               bool sync_upload) {
+int em_layout(const DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
+              int tileCols, double* trace, double* rows, const EmAux* aux, EmSizes& sz) {
     if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
-    if (&w == &ctx.em && ctx.emPending) {  // a sequence on another stream may still read it
-        EMSA_HIP_TRY(hipEventSynchronize(ctx.emDone));
-        ctx.emPending = false;
-    }
     w.regs.clear();
+    w.ordOff.clear();
+    sz = EmSizes{};
     // working order (Hilbert) and column-group boxes, except for the unrolled
     // forward, whose per-step checkpoints follow the caller's row order
     const bool order = (aux == nullptr);
-    size_t xpadF = 0, partF = 0, momD = 0, ordW = 0;
     long long tiles = 0, rowBlocks = 0;
-    std::vector<size_t> xoff(count), ooff(count);
     auto al64 = [](size_t v) { return (v + 63) / 64 * 64; };
     for (int k = 0; k < count; ++k) {
         const EmPair& pr = pairs[k];
@@ -1021,14 +1016,14 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
         } else {
             pl = plan_emicp(pr.N, pr.M, ctx.numCU, ctx.emBlocksPerCU);
         }
-        xoff[k] = xpadF;
-        xpadF += 3 * (size_t)pl.ldx;
+        const size_t xoff = sz.xpadF;
+        sz.xpadF += 3 * (size_t)pl.ldx;
         // per pair, in 4-byte words: boxes [ldx], Ys [3M], rowperm [M],
         // colperm [N], ordering scratch [4N + 4M]
-        ooff[k] = ordW;
+        const size_t ooff = sz.ordW;
         if (order)
-            ordW += al64(pl.ldx) + al64(3 * (size_t)pr.M) + al64(pr.M) + al64(pr.N) +
-                    al64(4 * ((size_t)pr.N + pr.M));
+            sz.ordW += al64(pl.ldx) + al64(3 * (size_t)pr.M) + al64(pr.M) + al64(pr.N) +
+                       al64(4 * ((size_t)pr.N + pr.M));
         for (int st = 0; st < n_starts; ++st) {
             EmRegDesc d{};
             d.X = pr.X;
@@ -1047,11 +1042,13 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
             const long long t = (long long)pl.rowBlocks * pl.chunks;
             tiles += t;
             rowBlocks += pl.rowBlocks;
-            // offsets for now, pointers below
-            d.partial = reinterpret_cast<float*>(partF);
-            partF += (size_t)t * 5 * kEmRows;
-            d.moments = reinterpret_cast<double*>(momD);
-            momD += (size_t)pl.rowBlocks * kMomPad;
+            // offsets for now (em_bind turns them into pointers)
+            d.Xpad = reinterpret_cast<float*>(xoff);
+            if (order) w.ordOff.push_back(ooff);
+            d.partial = reinterpret_cast<float*>(sz.partF);
+            sz.partF += (size_t)t * 5 * kEmRows;
+            d.moments = reinterpret_cast<double*>(sz.momD);
+            sz.momD += (size_t)pl.rowBlocks * kMomPad;
             d.trace = (count == 1 && n_starts == 1) ? trace : nullptr;
             d.rows = (count == 1 && n_starts == 1) ? rows : nullptr;
             d.ckpt = nullptr;
@@ -1071,20 +1068,18 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
         w.singleChunk = w.singleChunk && d.chunks == 1;
         w.someSingleChunk = w.someSingleChunk || d.chunks == 1;
     }
-    const int nregs = (int)w.regs.size();
-    int rc;
-    if ((rc = w.xpad.reserve(sizeof(float) * xpadF)) != EMSA_OK) return rc;
-    if ((rc = w.partial.reserve(sizeof(float) * partF)) != EMSA_OK) return rc;
-    if ((rc = w.moments.reserve(sizeof(double) * momD)) != EMSA_OK) return rc;
-    if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
-    if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
-    if (order && (rc = w.order.reserve(4 * ordW)) != EMSA_OK) return rc;
-    for (int i = 0; i < nregs; ++i) {
+    sz.nregs = (int)w.regs.size();
+    return EMSA_OK;
+}
+
+void em_bind(EmWork& w, float* xpad, float* partial, double* moments, double* state,
+             unsigned* order) {
+    auto al64 = [](size_t v) { return (v + 63) / 64 * 64; };
+    for (size_t i = 0; i < w.regs.size(); ++i) {
         EmRegDesc& d = w.regs[i];
-        const int k = d.slot / n_starts;
-        d.Xpad = w.xpad.as<float>() + xoff[k];
-        if (order) {
-            float* o = w.order.as<float>() + ooff[k];
+        d.Xpad = xpad + reinterpret_cast<size_t>(d.Xpad);
+        if (!w.ordOff.empty()) {
+            float* o = reinterpret_cast<float*>(order) + w.ordOff[i];
             d.gbox = o;
             o += al64(d.ldx);
             d.Ys = o;
@@ -1100,10 +1095,32 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
             d.rowperm = d.colperm = nullptr;
             d.sortTmp = nullptr;
         }
-        d.partial = w.partial.as<float>() + reinterpret_cast<size_t>(d.partial);
-        d.moments = w.moments.as<double>() + reinterpret_cast<size_t>(d.moments);
-        d.state = w.state.as<double>() + (size_t)i * kStateW;
+        d.partial = partial + reinterpret_cast<size_t>(d.partial);
+        d.moments = moments + reinterpret_cast<size_t>(d.moments);
+        d.state = state + i * kStateW;
     }
+}
+
+int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
+               int tileCols, double* trace, double* rows, hipStream_t s, const EmAux* aux,
+               bool sync_upload) {
+    if (count <= 0 || n_starts <= 0) return set_error("emicp: empty batch"), EMSA_EINVAL;
+    if (&w == &ctx.em && ctx.emPending) {  // a sequence on another stream may still read it
+        EMSA_HIP_TRY(hipEventSynchronize(ctx.emDone));
+        ctx.emPending = false;
+    }
+    EmSizes sz;
+    int rc = em_layout(ctx, w, pairs, count, n_starts, tileCols, trace, rows, aux, sz);
+    if (rc != EMSA_OK) return rc;
+    const int nregs = sz.nregs;
+    if ((rc = w.xpad.reserve(sizeof(float) * sz.xpadF)) != EMSA_OK) return rc;
+    if ((rc = w.partial.reserve(sizeof(float) * sz.partF)) != EMSA_OK) return rc;
+    if ((rc = w.moments.reserve(sizeof(double) * sz.momD)) != EMSA_OK) return rc;
+    if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
+    if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
+    if (sz.ordW > 0 && (rc = w.order.reserve(4 * sz.ordW)) != EMSA_OK) return rc;
+    em_bind(w, w.xpad.as<float>(), w.partial.as<float>(), w.moments.as<double>(),
+            w.state.as<double>(), sz.ordW > 0 ? w.order.as<unsigned>() : nullptr);
     EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(EmRegDesc) * nregs,
                                 hipMemcpyHostToDevice, s));
     if (sync_upload) EMSA_HIP_TRY(hipStreamSynchronize(s));  // w.regs may be rewritten next

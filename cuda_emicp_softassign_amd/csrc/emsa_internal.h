@@ -100,6 +100,8 @@ struct EmPair {
 struct EmWork {
     DevBuf xpad, partial, moments, state, desc, order;
     std::vector<EmRegDesc> regs;
+    std::vector<size_t> ordOff;  // per registration: its pair's offset in `order` (words);
+                                 // empty when the clouds keep the caller's order
     long long tiles = 0;       // affinity workgroups per step
     long long rowBlocks = 0;   // row blocks per step over the whole table
     bool singleChunk = false;      // every registration's tile spans its whole target
@@ -162,6 +164,8 @@ struct DeviceCtx {
 // Returns the context of the current HIP device (created on first use) or
 // nullptr with the error set.
 DeviceCtx* current_ctx();
+// The context of device dev if one was created, else nullptr (creates nothing).
+DeviceCtx* find_ctx(int dev);
 
 // The sigma^2 schedule in float, exactly as the reference loop runs it.
 std::vector<float> sigma_schedule(const emsa_params& p);
@@ -184,6 +188,19 @@ struct EmAux {
 };
 // sync_upload: wait for the descriptor upload before returning (callers that
 // rewrite w.regs before the stream has consumed it need this).
+// em_prepare = em_layout (descriptors with buffer offsets, and the buffer
+// sizes) + buffers of the sizes + em_bind (offsets to pointers) + upload.
+struct EmSizes {
+    size_t xpadF = 0;  // floats of the padded targets
+    size_t partF = 0;  // floats of the chunk partial sums
+    size_t momD = 0;   // doubles of the row-block moments
+    size_t ordW = 0;   // 4-byte words of the ordering buffers (0: caller's order)
+    int nregs = 0;     // registrations (kStateW doubles of state each)
+};
+int em_layout(const DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
+              int tileCols, double* trace, double* rows, const EmAux* aux, EmSizes& sz);
+void em_bind(EmWork& w, float* xpad, float* partial, double* moments, double* state,
+             unsigned* order);
 int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_starts,
                int tileCols, double* trace, double* rows, hipStream_t s,
                const EmAux* aux = nullptr, bool sync_upload = true);

@@ -141,6 +141,18 @@ int emsa_icp_nearest(const float *X, int N, const float *Y, int M, const float *
 int emsa_emicp_dev(const float *dX, int N, const float *dY, int M, float *dRt,
                    const emsa_params *p, void *stream);
 
+/* EM-ICP on device buffers that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, kernels and results as emsa_emicp_dev, launched directly on
+ * `stream` (never recorded into the library's own graphs). The workspace is a
+ * fixed 16 MiB device array per GPU, held until the process ends; clouds
+ * whose workspace does not fit (about N, M up to 65,536 points at the default
+ * decomposition) are rejected with EMSA_EINVAL. Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU race. */
+int emsa_emicp_dev_noalloc(const float *dX, int N, const float *dY, int M, float *dRt,
+                           const emsa_params *p, void *stream);
+
 /* ---- Batched EM-ICP (no counterpart in the reference) -------------------
  * count source/target pairs of uneven sizes, each registered from n_starts
  * initial transforms, all in one kernel launch per sigma^2 step (the same
