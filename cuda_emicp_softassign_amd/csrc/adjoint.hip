@@ -558,6 +558,7 @@ struct AdjWork {
     DevBuf xpad, adj, coef, colpart, rowpart, rtmom, desc;
     std::vector<AdjDesc> regs;
     long long tiles = 0, rowTiles = 0, cols = 0;
+    WorkFence fence;  // the last backward that used this workspace, on any stream
 };
 
 constexpr int kMaxDevices = 64;
@@ -657,6 +658,9 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
     AdjWork* wp = adj_work(*ctx);
     if (!wp) return set_error("unrolled_bwd: device index out of range"), EMSA_ENODEV;
     AdjWork& w = *wp;
+    // a backward on another stream may still read this workspace
+    if ((rc = w.fence.wait()) != EMSA_OK) return rc;
+    FenceMark mark(w.fence, s);
     std::vector<size_t> offs;
     ckpt_doubles(pairs, count, steps, &offs);
     const int tileCols = 1024;

@@ -74,6 +74,15 @@ int DevBuf::reserve(size_t want) {
     return EMSA_OK;
 }
 
+int WorkFence::wait() {
+    if (!ev) EMSA_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (pending) {
+        EMSA_HIP_TRY(hipEventSynchronize(ev));
+        pending = false;
+    }
+    return EMSA_OK;
+}
+
 DeviceCtx* current_ctx() {
     int dev = 0, ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {

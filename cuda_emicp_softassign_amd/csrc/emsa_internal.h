@@ -42,6 +42,28 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Completion of the last work, on whatever stream it ran, that used a
+// workspace shared by calls on one device. A call that rewrites the
+// workspace first waits for it (wait), then marks its own work (a FenceMark
+// records the event on the call's stream when it goes out of scope, so every
+// return path marks what was already enqueued).
+struct WorkFence {
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    int wait();  // EMSA_OK / EMSA_EHIP
+};
+struct FenceMark {
+    WorkFence& f;
+    hipStream_t s;
+    FenceMark(WorkFence& f_, hipStream_t s_) : f(f_), s(s_) {}
+    FenceMark(const FenceMark&) = delete;
+    FenceMark& operator=(const FenceMark&) = delete;
+    ~FenceMark() {
+        if (f.ev && hipEventRecord(f.ev, s) == hipSuccess) f.pending = true;
+        else (void)hipGetLastError();
+    }
+};
+
 // State of the EM-ICP loop, in doubles, one slot of kStateW per registration.
 constexpr int kStateW = 32;    // R[9] t[3] Xsum[3] status ...
 constexpr int kStR = 0, kStT = 9, kStXsum = 12, kStStatus = 15, kStResid = 16, kStLambda = 17;

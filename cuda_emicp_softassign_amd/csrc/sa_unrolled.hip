@@ -1166,6 +1166,7 @@ struct SauWork {
     DevBuf xmax;
     DevBuf P, state, u, v, orow, ocol, bcol, wfin, cpart, rmom, cby;
     DevBuf gU, gV, gOr, gOc, L, Q, zr, zc, adj, fcol, frow, fmomr, fmomc;
+    WorkFence fence;  // the last forward or backward that used it, on any stream
 };
 constexpr int kMaxDevices = 64;
 SauWork* g_sau[kMaxDevices];  // per device, created under that device's lock, never freed
@@ -1342,10 +1343,13 @@ int emsa_softassign_unrolled_fwd(const float* dX, const float* dY, int B, int N,
     std::lock_guard<std::mutex> lk(ctx->mu);
     SauWork* wp = sau_work(*ctx);
     if (!wp) return set_error("softassign_unrolled: device index out of range"), EMSA_ENODEV;
+    hipStream_t s = (hipStream_t)stream;
+    // a call on another stream may still use this workspace
+    if ((rc = wp->fence.wait()) != EMSA_OK) return rc;
+    FenceMark mark(wp->fence, s);
     Sau d;
     sau_dims(d, a, ctx->numCU);
     if ((rc = sau_bind_forward(d, *wp, dX, dY, dRt0, dckpt)) != EMSA_OK) return rc;
-    hipStream_t s = (hipStream_t)stream;
     const dim3 blk(kWave * kWG);
     const unsigned ub = (unsigned)B;
     hipLaunchKernelGGL(sau_init_kernel, dim3((unsigned)((std::max(N, d.ldk) + 255) / 256), ub),
@@ -1395,6 +1399,10 @@ int emsa_softassign_unrolled_bwd(const float* dX, const float* dY, int B, int N,
     SauWork* wp = sau_work(*ctx);
     if (!wp) return set_error("softassign_unrolled: device index out of range"), EMSA_ENODEV;
     SauWork& w = *wp;
+    hipStream_t s = (hipStream_t)stream;
+    // a call on another stream may still use this workspace
+    if ((rc = w.fence.wait()) != EMSA_OK) return rc;
+    FenceMark mark(w.fence, s);
     Sau d;
     sau_dims(d, a, ctx->numCU);
     if ((rc = sau_bind_forward(d, w, dX, dY, dRt0, const_cast<void*>(dckpt))) != EMSA_OK)
@@ -1431,7 +1439,6 @@ int emsa_softassign_unrolled_bwd(const float* dX, const float* dY, int B, int N,
     d.fmomr = w.fmomr.as<double>();
     d.fmomc = w.fmomc.as<double>();
 
-    hipStream_t s = (hipStream_t)stream;
     const dim3 blk(kWave * kWG);
     const unsigned ub = (unsigned)B;
     const int maxv = std::max({3 * N, 3 * M, d.ldk});

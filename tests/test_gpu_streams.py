@@ -1,0 +1,82 @@
+"""Unrolled (differentiable) registrations issued on two torch streams at once.
+
+The unrolled forward and backward entry points keep one workspace per device
+and return while their kernels are still queued on the caller's stream. A
+call on another stream waits for the previous call's work before it rewrites
+that workspace. Bar: gradients and results of two problems run on two streams
+without synchronising in between are bit-identical to the same calls made one
+after the other on one stream.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cuda_emicp_softassign_amd.clouds import range_scans, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def UR():
+    from cuda_emicp_softassign_amd import unrolled
+    return unrolled
+
+
+def _em_problem(n, m, seed, B):
+    Xs, Ys = [], []
+    for b in range(B):
+        X, Y, _, _ = range_scans(n, m, seed=seed + b)
+        Xs.append(X)
+        Ys.append(Y)
+    return np.stack(Xs).astype(np.float32), np.stack(Ys).astype(np.float32)
+
+
+def _sa_problem(n, m, seed, B):
+    Xs, Ys = [], []
+    for b in range(B):
+        X, Y, _, _ = uniform_cube(max(n, m), seed=seed + b)
+        Xs.append(X[:n])
+        Ys.append(Y[:m])
+    return np.stack(Xs).astype(np.float32), np.stack(Ys).astype(np.float32)
+
+
+def _fwd_bwd(run, X, Y):
+    Xd = torch.tensor(X, device="cuda").requires_grad_(True)
+    Yd = torch.tensor(Y, device="cuda").requires_grad_(True)
+    Rt = run(Xd, Yd)
+    (Rt.double() ** 2).sum().backward()
+    return Rt, Xd, Yd
+
+
+def _gather(res):
+    Rt, Xd, Yd = res
+    return [Rt.detach().cpu().numpy(), Xd.grad.cpu().numpy(), Yd.grad.cpu().numpy()]
+
+
+def _two_streams(run, probs):
+    torch.cuda.synchronize()
+    seq = [_gather(_fwd_bwd(run, X, Y)) for X, Y in probs]
+    torch.cuda.synchronize()
+    main = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream() for _ in probs]
+    out = []
+    for s, (X, Y) in zip(streams, probs):
+        s.wait_stream(main)
+        with torch.cuda.stream(s):
+            out.append(_fwd_bwd(run, X, Y))  # no synchronisation between the two
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(seq, [_gather(r) for r in out])):
+        for name, x, y in zip(("Rt", "gX", "gY"), a, b):
+            assert np.array_equal(x, y), (k, name, np.abs(x - y).max())
+
+
+def test_emicp_unrolled_two_streams(UR):
+    probs = [_em_problem(700, 650, 300, 2), _em_problem(1100, 900, 310, 3)]
+    print("emicp unrolled on two streams: B=2 700x650, B=3 1100x900", flush=True)
+    _two_streams(lambda X, Y: UR.emicp_unrolled(X, Y, sigma2_0=0.01, steps=12), probs)
+
+
+def test_softassign_unrolled_two_streams(UR):
+    probs = [_sa_problem(400, 380, 320, 2), _sa_problem(600, 520, 330, 1)]
+    print("softassign unrolled on two streams: B=2 400x380, B=1 600x520", flush=True)
+    _two_streams(lambda X, Y: UR.softassign_unrolled(X, Y, JMAX=6, I0=1, I1=3), probs)
