@@ -26,7 +26,8 @@ from ._lib import (EMSA_OK, SOFTASSIGN_LOGDOMAIN, SOFTASSIGN_MATERIALISED, TRACE
 
 __all__ = [
     "registrationParameters", "emicp", "softassign", "findRTfromS", "cloud2data", "printRT",
-    "emicp_traced", "emicp_step", "softassign_traced", "emicp_iterations", "emicp_dev",
+    "emicp_traced", "emicp_step", "softassign_traced", "softassign_step", "emicp_iterations",
+    "emicp_dev",
     "EmsaError", "SOFTASSIGN_MATERIALISED", "SOFTASSIGN_LOGDOMAIN", "TRACE_W",
     "emicp_batched", "emicp_batched_dev", "autotune_emicp", "multistart_rt0", "select_best",
     "graph_cache_clear", "set_tiling",
@@ -241,6 +242,33 @@ def softassign_traced(cloud_target, cloud_source, h_R, h_t, param=None, variant=
         n))
     Rr, tt = _writeback(h_R, h_t, R, t)
     return Rr, tt, tr[:rc]
+
+
+def softassign_step(cloud_target, cloud_source, h_R, h_t, T, alpha, I1, o_row, o_col,
+                    variant="materialised"):
+    """One Softassign inner iteration on the GPU (src/softassign.cu:494-818)
+    from (h_R, h_t) at temperature T, with the persistent slack vectors o_row
+    (N) and o_col (M) (float32 arrays, updated in place). Returns dict(R, t,
+    a (I1, N) and b (I1, M): every Sinkhorn sweep's row and column sums before
+    their division, slack included; s (N): the post-Sinkhorn row sums;
+    record: T, Sigma, Xc(3), Yc(3), S(9), R(9), t(3))."""
+    X, N = cloud2data(cloud_target)
+    Y, M = cloud2data(cloud_source)
+    R, t = _inout(h_R, h_t)
+    for v, n in ((o_row, N), (o_col, M)):
+        if not (isinstance(v, np.ndarray) and v.dtype == np.float32 and v.size == n
+                and v.flags.c_contiguous):
+            raise ValueError("o_row / o_col: contiguous float32 arrays of N / M entries")
+    a = np.zeros((I1, N))
+    b = np.zeros((I1, M))
+    sr = np.zeros(N)
+    rec = np.zeros(TRACE_W)
+    _lib.check("softassign_step", _lib.load().emsa_softassign_step(
+        _fptr(X), N, _fptr(Y), M, _fptr(R), _fptr(t), float(T), float(alpha), int(I1),
+        _fptr(o_row), _fptr(o_col), _variant(variant), _dptr(a), _dptr(b), _dptr(sr),
+        _dptr(rec)))
+    Rr, tt = _writeback(h_R, h_t, R, t)
+    return dict(R=Rr, t=tt, a=a, b=b, s=sr, record=rec)
 
 
 def findRTfromS(h_Xc, h_Yc, h_S, h_R=None, h_t=None):

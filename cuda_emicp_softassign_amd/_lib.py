@@ -62,6 +62,9 @@ SIGNATURES = {
     "emsa_softassign": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int]),
     "emsa_softassign_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int,
                                          _dp, C.c_int]),
+    "emsa_softassign_step": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float,
+                                       C.c_float, C.c_int, _fp, _fp, C.c_int, _dp, _dp, _dp,
+                                       _dp]),
     "emsa_find_rt_from_s": (C.c_int, [_fp, _fp, _fp, _fp, _fp]),
     "emsa_icp": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp]),
     "emsa_icp_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, _dp, C.c_int]),

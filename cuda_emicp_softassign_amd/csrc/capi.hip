@@ -291,6 +291,90 @@ static int softassign_host(const float* X, int N, const float* Y, int M, float* 
     return EMSA_OK;
 }
 
+// One inner iteration of Softassign on host buffers (emsa_softassign_step).
+static int softassign_step_host(const float* X, int N, const float* Y, int M, float* R, float* t,
+                                float T, float alpha, int I1, float* o_row, float* o_col,
+                                int variant, double* ha, double* hb, double* hs, double* record) {
+    if (!X || !Y || !R || !t || !o_row || !o_col || N <= 0 || M <= 0) {
+        set_error("softassign_step: null pointer or empty cloud");
+        return EMSA_EINVAL;
+    }
+    if (I1 < 1) return set_error("softassign_step: I1 must be >= 1"), EMSA_EINVAL;
+    if (variant != EMSA_SOFTASSIGN_MATERIALISED && variant != EMSA_SOFTASSIGN_LOGDOMAIN)
+        return set_error("softassign_step: unknown variant"), EMSA_EINVAL;
+    if (!(T > 0.f) || !std::isfinite(T) || !std::isfinite(alpha) || !finite_cloud(X, N) ||
+        !finite_cloud(Y, M))
+        return set_error("softassign_step: non-finite input or T <= 0"), EMSA_ENONFINITE;
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(R[i])) return set_error("softassign_step: non-finite R"), EMSA_ENONFINITE;
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(t[i])) return set_error("softassign_step: non-finite t"), EMSA_ENONFINITE;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    int rc;
+    const size_t nTap = (size_t)I1 * N + (size_t)I1 * M + (size_t)N;
+    if ((rc = ctx->X.reserve(sizeof(float) * 3 * (size_t)N)) != EMSA_OK) return rc;
+    if ((rc = ctx->Y.reserve(sizeof(float) * 3 * (size_t)M)) != EMSA_OK) return rc;
+    if ((rc = ctx->rt.reserve(sizeof(float) * 32)) != EMSA_OK) return rc;
+    if ((rc = ctx->trace.reserve(sizeof(double) * EMSA_TRACE_W)) != EMSA_OK) return rc;
+    if ((rc = ctx->sastep.reserve(sizeof(double) * nTap + sizeof(float) * ((size_t)N + M))) !=
+        EMSA_OK)
+        return rc;
+    double* dA = ctx->sastep.as<double>();
+    double* dB = dA + (size_t)I1 * N;
+    double* dS = dB + (size_t)I1 * M;
+    float* dOrow = reinterpret_cast<float*>(dA + nTap);
+    float* dOcol = dOrow + N;
+    float rt[12];
+    std::memcpy(rt, R, 36);
+    std::memcpy(rt + 9, t, 12);
+    float* drt = ctx->rt.as<float>();
+    EMSA_HIP_TRY(hipMemcpyAsync(ctx->X.ptr, X, sizeof(float) * 3 * (size_t)N,
+                                hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(ctx->Y.ptr, Y, sizeof(float) * 3 * (size_t)M,
+                                hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(drt, rt, sizeof(rt), hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(dOrow, o_row, sizeof(float) * N, hipMemcpyHostToDevice, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(dOcol, o_col, sizeof(float) * M, hipMemcpyHostToDevice, s));
+    double xmax2 = 0.0;
+    for (int r = 0; r < N; ++r) {
+        const double a = X[r], b = X[(size_t)N + r], c = X[2 * (size_t)N + r];
+        xmax2 = std::max(xmax2, a * a + b * b + c * c);
+    }
+    emsa_params p;
+    emsa_default_params(&p);
+    p.I1 = I1;
+    p.alpha = alpha;
+    const SaStep st{T, dOrow, dOcol, dA, dB, dS};
+    rc = softassign_device(*ctx, ctx->X.as<float>(), N, ctx->Y.as<float>(), M, drt, drt + 16, p,
+                           variant, s, ctx->trace.as<double>(), (float)xmax2, &st);
+    if (rc != EMSA_OK) return rc;
+    float out[12];
+    std::vector<double> taps(nTap), rec(EMSA_TRACE_W);
+    std::vector<float> orow(N), ocol(M);
+    EMSA_HIP_TRY(hipMemcpyAsync(out, drt + 16, sizeof(out), hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(taps.data(), dA, sizeof(double) * nTap, hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(rec.data(), ctx->trace.ptr, sizeof(double) * EMSA_TRACE_W,
+                                hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(orow.data(), dOrow, sizeof(float) * N, hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipMemcpyAsync(ocol.data(), dOcol, sizeof(float) * M, hipMemcpyDeviceToHost, s));
+    EMSA_HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < 12; ++i)
+        if (!std::isfinite(out[i]))
+            return set_error("softassign_step: non-finite result"), EMSA_ENONFINITE;
+    std::memcpy(R, out, 36);
+    std::memcpy(t, out + 9, 12);
+    std::memcpy(o_row, orow.data(), sizeof(float) * N);
+    std::memcpy(o_col, ocol.data(), sizeof(float) * M);
+    if (ha) std::memcpy(ha, taps.data(), sizeof(double) * I1 * (size_t)N);
+    if (hb) std::memcpy(hb, taps.data() + (size_t)I1 * N, sizeof(double) * I1 * (size_t)M);
+    if (hs) std::memcpy(hs, taps.data() + (size_t)I1 * (N + M), sizeof(double) * N);
+    if (record) std::memcpy(record, rec.data(), sizeof(double) * EMSA_TRACE_W);
+    return EMSA_OK;
+}
+
 }  // namespace emsa
 
 using namespace emsa;
@@ -373,6 +457,14 @@ int emsa_softassign_traced(const float* X, int N, const float* Y, int M, float* 
     const int rc = softassign_host(X, N, Y, M, R, t, *p, variant, trace, trace_cap);
     const int nrec = std::max(0, p->JMAX) * std::max(0, p->I0);
     return rc == EMSA_OK ? std::min(nrec, trace_cap) : rc;
+}
+
+int emsa_softassign_step(const float* X, int N, const float* Y, int M, float* R, float* t,
+                         float T, float alpha, int I1, float* o_row, float* o_col, int variant,
+                         double* a, double* b, double* s, double* record) {
+    clear_error();
+    return softassign_step_host(X, N, Y, M, R, t, T, alpha, I1, o_row, o_col, variant, a, b, s,
+                                record);
 }
 
 int emsa_find_rt_from_s(const float* Xc, const float* Yc, const float* S, float* R, float* t) {

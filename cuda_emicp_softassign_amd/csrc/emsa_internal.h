@@ -154,8 +154,9 @@ struct DeviceCtx {
     // workspaces of the eager (non-graph) entry points
     DevBuf X, Y, rt, trace, rows, misc, resid;
     EmWork em;
-    // softassign workspaces
-    DevBuf K, u, v, orow, ocol, colpart, samom, sastate;
+    // softassign workspaces (sastep: staging of emsa_softassign_step's slack
+    // vectors and taps)
+    DevBuf K, u, v, orow, ocol, colpart, samom, sastate, sastep;
     // ICP workspaces: per-tile nearest candidates, row-block moments, state,
     // per-row neighbour and distance (the correspondence query)
     DevBuf icpDist, icpIdx, icpMom, icpState, icpRowIdx, icpRowDist;
@@ -254,11 +255,25 @@ int emicp_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                  const float* dRtIn, float* dRtOut, const emsa_params& p,
                  const std::vector<float>& sigmas, hipStream_t s, double* trace, double* rows);
 
+// One inner iteration of Softassign instead of the whole schedule
+// (emsa_softassign_step): at temperature T, from the slack vectors orow [N]
+// and ocol [M] (device, in/out: read after set-up, written back at the end),
+// with the per-sweep taps a [I1][N], b [I1][M] and the post-Sinkhorn row sums
+// s [N] (device, each nullable).
+struct SaStep {
+    float T;
+    float* orow;
+    float* ocol;
+    double* a;
+    double* b;
+    double* s;
+};
+
 // Softassign on device buffers (see softassign.hip). xmax2: max |x|^2 over
 // the target cloud (chooses the log domain's form of the exponent).
 int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                       const float* dRtIn, float* dRtOut, const emsa_params& p, int variant,
-                      hipStream_t s, double* trace, float xmax2);
+                      hipStream_t s, double* trace, float xmax2, const SaStep* step = nullptr);
 
 // Horn Procrustes on the device for one (Xc, Yc, S) (all float in, float out).
 int find_rt_device(DeviceCtx& ctx, const float* hXc, const float* hYc, const float* hS,

@@ -107,6 +107,12 @@ struct SaBufs {
     double* cby;      // [3][ldk / 64] sum_c b_c y_c per finalize block
     double* state;    // kStateW
     double* trace;    // nullable
+    // nullable taps of one Sinkhorn sweep (emsa_softassign_step): the row sums
+    // a_r [N] and column sums b_c [M] before normalisation, slack included, and
+    // the post-Sinkhorn row sums s_r [N] of the final pass
+    double* tapA;
+    double* tapB;
+    double* tapS;
 };
 
 __global__ void __launch_bounds__(1024)
@@ -303,6 +309,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
             const float a = scale * tot[j] + b.orow[r];
             b.orow[r] = b.orow[r] / a;
             b.u[r] = LOG ? (su - log2f(a)) : (su / a);
+            if (b.tapA) b.tapA[r] = (double)a;
         }
         return;
     }
@@ -319,6 +326,7 @@ sa_rows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         const double s = scale * tot[j * Q];
         const double D[3] = {scale * tot[j * Q + 1], scale * tot[j * Q + 2],
                              scale * tot[j * Q + 3]};
+        if (b.tapS) b.tapS[r] = s;
         const double x[3] = {b.X[r], b.X[(size_t)N + r], b.X[2 * (size_t)N + r]};
         mm[j][0] = s;
 #pragma unroll
@@ -459,6 +467,7 @@ sa_rowcol_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
         for (int w2 = 1; w2 < NW; ++w2) tot += red[w2][j];
         const float su = FIRST ? 1.f : b.u[r];
         const float a = su * tot + b.orow[r];
+        if (b.tapA) b.tapA[r] = (double)a;
         b.orow[r] = b.orow[r] / a;
         const float un = su / a;
         b.u[r] = un;
@@ -600,6 +609,7 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
         const float sv = FIRST ? (LOG ? 0.f : 1.f) : b.v[cc];
         const float scale = LOG ? exp2f(sv) : sv;
         const float bc = (float)(scale * tot) + b.ocol[cc];
+        if (b.tapB) b.tapB[cc] = (double)bc;
         b.ocol[cc] = b.ocol[cc] / bc;
         const float vn = LOG ? (sv - log2f(bc)) : (sv / bc);
         b.v[cc] = vn;
@@ -777,7 +787,7 @@ int launch_rowcol(bool first, int slabs, dim3 grid, dim3 blk, hipStream_t s, con
 
 int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, int M,
                       const float* dRtIn, float* dRtOut, const emsa_params& p, int variant,
-                      hipStream_t s, double* trace, float xmax2) {
+                      hipStream_t s, double* trace, float xmax2, const SaStep* step) {
     const bool LOGV = (variant == EMSA_SOFTASSIGN_LOGDOMAIN);
     const SaPlan pl = plan_sa(N, M, ctx.numCU);
     // moment rows of the final pass (one per final-pass workgroup)
@@ -825,13 +835,20 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
                        p.moutlier);
     EMSA_HIP_TRY(hipGetLastError());
+    if (step) {  // the caller's slack state instead of moutlier
+        EMSA_HIP_TRY(hipMemcpyAsync(b.orow, step->orow, sizeof(float) * N,
+                                    hipMemcpyDeviceToDevice, s));
+        EMSA_HIP_TRY(hipMemcpyAsync(b.ocol, step->ocol, sizeof(float) * M,
+                                    hipMemcpyDeviceToDevice, s));
+    }
+    const int JMAX = step ? 1 : p.JMAX, I0 = step ? 1 : p.I0;
     const dim3 rgrid((unsigned)pl.rowBlocks), cgrid((unsigned)(pl.colBlocks * pl.H)),
         fgrid((unsigned)finBlocks);
     const dim3 blk(kWave * kSaWaves), wblk(kWave * kSaFinWavesWide);
     const dim3 xgrid((unsigned)pl.fuseBlocks), xblk(kWave * kSaFuseWaves);
-    float T = p.T_0;
+    float T = step ? step->T : p.T_0;
     int it = 0;
-    for (int Titer = 1; Titer <= p.JMAX; ++Titer) {
+    for (int Titer = 1; Titer <= JMAX; ++Titer) {
         const double k1d = kLog2e / (double)T;
         const float k1 = (float)k1d;
         const float k0 = (float)((double)p.alpha * k1d - 0.5 * std::log2((double)T));
@@ -840,9 +857,13 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             if (expand) launch_log_rows<true>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
             else launch_log_rows<false>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
         };
-        for (int i0 = 0; i0 < p.I0; ++i0, ++it) {
+        for (int i0 = 0; i0 < I0; ++i0, ++it) {
             for (int sw = 0; sw < p.I1; ++sw) {
                 const int last = (sw + 1 == p.I1) ? 1 : 0;
+                if (step) {  // this sweep's taps
+                    b.tapA = step->a ? step->a + (size_t)sw * N : nullptr;
+                    b.tapB = step->b ? step->b + (size_t)sw * M : nullptr;
+                }
                 if (LOGV) {
                     log_rows(sw == 0 ? 0 : 1);
                     if (expand)
@@ -885,17 +906,26 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                 }
                 EMSA_HIP_TRY(hipGetLastError());
             }
+            b.tapA = b.tapB = nullptr;
+            b.tapS = step ? step->s : nullptr;
             if (LOGV)
                 log_rows(2);
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2, kSaFinalRows, kSaFinalWaves>),
                                    dim3((unsigned)finalBlocks), dim3(kWave * kSaFinalWaves), 0, s,
                                    b, N, M, pl.ldk, k1, k0);
+            b.tapS = nullptr;
             hipLaunchKernelGGL(sa_solve_kernel, dim3(1), dim3(kWave * kSaSolveWaves), 0, s, b,
                                nmom, finBlocks, T, it);
             EMSA_HIP_TRY(hipGetLastError());
         }
         T = T * p.TFACTOR;  // softassign.cu:820, float
+    }
+    if (step) {
+        EMSA_HIP_TRY(hipMemcpyAsync(step->orow, b.orow, sizeof(float) * N,
+                                    hipMemcpyDeviceToDevice, s));
+        EMSA_HIP_TRY(hipMemcpyAsync(step->ocol, b.ocol, sizeof(float) * M,
+                                    hipMemcpyDeviceToDevice, s));
     }
     hipLaunchKernelGGL(sa_output_kernel, dim3(1), dim3(64), 0, s, b.state, dRtOut);
     EMSA_HIP_TRY(hipGetLastError());

@@ -101,6 +101,21 @@ int emsa_softassign(const float *X, int N, const float *Y, int M, float *R, floa
 int emsa_softassign_traced(const float *X, int N, const float *Y, int M, float *R, float *t,
                            const emsa_params *p, int variant, double *trace, int trace_cap);
 
+/* One inner iteration of Softassign (src/softassign.cu:494-818) from (R, t)
+ * at temperature T: the affinities (:106-131), I1 Sinkhorn sweeps with the
+ * slack vectors o_row [N] and o_col [M] (in/out: they persist across the
+ * reference's whole run, :385-390, and are divided by every sweep's sums),
+ * the post-Sinkhorn sums, the weighted centroids, S and Horn's step; R, t
+ * updated. Per-sweep views for parity checks (each nullable):
+ * a [I1][N]: row sums a_r = sum_c M_rc + o_row[r] before the row division
+ * (:538-552); b [I1][M]: column sums b_c = sum_r M_rc + o_col[c] before the
+ * column division (:579-590); s [N]: post-Sinkhorn row sums s_r (:626-639);
+ * record: T, Sigma = sum s_r, Xc[3], Yc[3], S[9], R[9], t[3] as in
+ * emsa_softassign_traced. variant: EMSA_SOFTASSIGN_*. */
+int emsa_softassign_step(const float *X, int N, const float *Y, int M, float *R, float *t,
+                         float T, float alpha, int I1, float *o_row, float *o_col, int variant,
+                         double *a, double *b, double *s, double *record);
+
 /* Horn's closed-form Procrustes step, computed on the device in FP64.
  * Replaces findRTfromS (src/findRTfromS.cpp:33-97; declared at
  * src/3dregistration.h:104-107). S is h_S of the reference: S[3a+b] =

@@ -171,12 +171,14 @@ double CAT(or_emicp_step_, SFX)(const REAL *X, int N, const REAL *Y, int M, REAL
 /* o_row[N] and o_col[M] persist across the whole run (softassign.cu:385-390). */
 /* Outputs: rowsum s_r after Sinkhorn (softassign.cu:626), colsum b_c (the    */
 /* last pre-normalisation column sums including o_col, softassign.cu:579-590),*/
-/* Xc, Yc, S; R, t are updated in place. Returns sumM.                        */
+/* Xc, Yc, S; R, t are updated in place. Returns sumM. asw [I1][N] and      */
+/* bsw [I1][M] (nullable) receive every sweep's row sums a_r and column sums  */
+/* b_c before their division, slack included (softassign.cu:538-552, :579-590).*/
 /* ------------------------------------------------------------------------- */
 REAL CAT(or_softassign_inner_, SFX)(const REAL *X, int N, const REAL *Y, int M, REAL *R, REAL *t,
                                     float T_cur, float alpha, int I1, REAL *o_row, REAL *o_col,
                                     REAL *Mat, REAL *rowsum, REAL *colsum, REAL *Xc, REAL *Yc,
-                                    REAL *S)
+                                    REAL *S, REAL *asw, REAL *bsw)
 {
     const REAL *Xx = X, *Xy = X + N, *Xz = X + 2 * (size_t)N;
     const REAL *Yx = Y, *Yy = Y + M, *Yz = Y + 2 * (size_t)M;
@@ -209,6 +211,7 @@ REAL CAT(or_softassign_inner_, SFX)(const REAL *X, int N, const REAL *Y, int M, 
             for (int c = 0; c < M; c++) s += Mat[(size_t)r * M + c];
             s += o_row[r]; /* saxpy, :552 */
             rowsum[r] = s;
+            if (asw) asw[(size_t)k * N + r] = s;
             for (int c = 0; c < M; c++) Mat[(size_t)r * M + c] /= s; /* :162 */
             o_row[r] /= s;                                           /* :164 */
         }
@@ -216,6 +219,8 @@ REAL CAT(or_softassign_inner_, SFX)(const REAL *X, int N, const REAL *Y, int M, 
         for (int r = 0; r < N; r++)
             for (int c = 0; c < M; c++) colsum[c] += Mat[(size_t)r * M + c];
         for (int c = 0; c < M; c++) colsum[c] += o_col[c]; /* :590 */
+        if (bsw)
+            for (int c = 0; c < M; c++) bsw[(size_t)k * M + c] = colsum[c];
 #pragma omp parallel for schedule(static)
         for (int r = 0; r < N; r++)
             for (int c = 0; c < M; c++) Mat[(size_t)r * M + c] /= colsum[c]; /* :193 */
@@ -308,7 +313,7 @@ int CAT(or_softassign_, SFX)(const REAL *X, int N, const REAL *Y, int M, REAL *R
         for (int i0 = 0; i0 < I0; i0++) {
             REAL Xc[3], Yc[3], S[9];
             REAL sumM = CAT(or_softassign_inner_, SFX)(X, N, Y, M, R, t, T_cur, alpha, I1, o_row,
-                                                      o_col, Mat, rs, cs, Xc, Yc, S);
+                                                      o_col, Mat, rs, cs, Xc, Yc, S, NULL, NULL);
             if (trace) {
                 double *o = trace + (size_t)it * OR_TRACE_SOFTASSIGN;
                 o[0] = T_cur;

@@ -62,7 +62,7 @@ def lib():
             f.restype = C.c_int
             f = getattr(L, "or_softassign_inner_" + sfx)
             f.argtypes = [p, C.c_int, p, C.c_int, p, p, C.c_float, C.c_float, C.c_int, p, p, p,
-                          p, p, p, p, p]
+                          p, p, p, p, p, C.c_void_p, C.c_void_p]
             f.restype = C.c_double if sfx == "f64" else C.c_float
         _lib = L
     return _lib
@@ -168,7 +168,9 @@ def softassign(X, Y, R0=None, t0=None, JMAX=100, I0=5, I1=3, alpha=3.0, T_0=100.
 
 def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col, precision="f64"):
     """One inner iteration; o_row/o_col are updated in place (pass float arrays
-    of the chosen precision). Returns dict(rowsum, colsum, Xc, Yc, S, R, t, sumM)."""
+    of the chosen precision). Returns dict(rowsum (post-Sinkhorn s_r), colsum
+    (last sweep's b_c), a (I1, N) and b (I1, M): every sweep's row and column
+    sums before their division, Xc, Yc, S, R, t, sumM)."""
     dt = _dt(precision)
     N, M = len(X), len(Y)
     R = np.ascontiguousarray(np.asarray(R).reshape(9), dtype=dt).copy()
@@ -179,11 +181,14 @@ def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col, precision="f64"):
     Xc = np.zeros(3, dt)
     Yc = np.zeros(3, dt)
     S = np.zeros(9, dt)
+    asw = np.zeros(I1 * N, dt)
+    bsw = np.zeros(I1 * M, dt)
     assert o_row.dtype == dt and o_col.dtype == dt
     sumM = getattr(lib(), "or_softassign_inner_" + precision)(
-        soa(X, dt), N, soa(Y, dt), M, R, t, T, alpha, I1, o_row, o_col, Mat, rs, cs, Xc, Yc, S)
+        soa(X, dt), N, soa(Y, dt), M, R, t, T, alpha, I1, o_row, o_col, Mat, rs, cs, Xc, Yc, S,
+        asw.ctypes.data, bsw.ctypes.data)
     return dict(rowsum=rs, colsum=cs, Xc=Xc, Yc=Yc, S=S, R=R.reshape(3, 3), t=t, sumM=sumM,
-                Mat=Mat.reshape(N, M))
+                Mat=Mat.reshape(N, M), a=asw.reshape(I1, N), b=bsw.reshape(I1, M))
 
 
 def icp_nearest(X, P):

@@ -77,14 +77,17 @@ def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col, eig):
             tz = X[r][2] - (R[6] * Y[c][0] + R[7] * Y[c][1] + R[8] * Y[c][2] + t[2])
             Mm[c * pitch + r] = math.exp(-((tx * tx + ty * ty + tz * tz) - alpha) / T) / math.sqrt(T)
     colsum = None
+    a_sweeps, b_sweeps = [], []
     for _ in range(I1):
         rowsum = [sum(Mm[c * pitch + r] for c in range(colsM)) + o_row[r] for r in range(rowsM)]
+        a_sweeps.append(list(rowsum))
         for c in range(colsM):
             for r in range(rowsM):
                 Mm[c * pitch + r] /= rowsum[r]
         for r in range(rowsM):
             o_row[r] /= rowsum[r]
         colsum = [sum(Mm[c * pitch + r] for r in range(rowsM)) + o_col[c] for c in range(colsM)]
+        b_sweeps.append(list(colsum))
         for c in range(colsM):
             for r in range(rowsM):
                 Mm[c * pitch + r] /= colsum[c]
@@ -105,4 +108,5 @@ def softassign_inner(X, Y, R, t, T, alpha, I1, o_row, o_col, eig):
         for i in range(3):
             S[i + 3 * j] = sum(Xr[r][i] * D[r][j] for r in range(rowsM))
     Rn, tn = _horn(Xc, Yc, S, eig)
-    return dict(rowsum=rowsum, colsum=colsum, sumM=sumM, Xc=Xc, Yc=Yc, S=S, R=Rn, t=tn)
+    return dict(rowsum=rowsum, colsum=colsum, sumM=sumM, Xc=Xc, Yc=Yc, S=S, R=Rn, t=tn,
+                a=a_sweeps, b=b_sweeps)

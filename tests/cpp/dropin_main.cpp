@@ -65,12 +65,25 @@ int main() {
     std::printf("findRTfromS max |error| %g\n", err2);
     if (!(err2 < 1e-5)) return 3;
 
-    float s_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, s_t[3] = {0, 0, 0};
-    param.T_0 = 0.5f; param.TFACTOR = 0.9f; param.alpha = 0.05f; param.JMAX = 40;
-    softassign(target, source, s_R, s_t, param);
-    printRT(s_R, s_t);
-    for (int i = 0; i < 9; ++i)
-        if (!std::isfinite(s_R[i])) return 4;
+    // softassign(), both variants: no slack (moutlier 0) and Sinkhorn run to
+    // convergence (I1 = 30), annealed until second-nearest partners no longer
+    // count, so the soft correspondences become the exact permutation and the
+    // known transform comes back
+    const int variants[2] = {EMSA_SOFTASSIGN_MATERIALISED, EMSA_SOFTASSIGN_LOGDOMAIN};
+    for (int v = 0; v < 2; ++v) {
+        float s_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, s_t[3] = {0, 0, 0};
+        param.T_0 = 0.5f; param.TFACTOR = 0.8f; param.alpha = 0.0f; param.JMAX = 48;
+        param.I0 = 2; param.I1 = 30; param.moutlier = 0.0f;
+        emsa_set_softassign_variant(variants[v]);
+        softassign(target, source, s_R, s_t, param);
+        printRT(s_R, s_t);
+        double err4 = 0;
+        for (int i = 0; i < 9; ++i) err4 = std::fmax(err4, std::fabs(s_R[i] - Rt[i]));
+        for (int i = 0; i < 3; ++i) err4 = std::fmax(err4, std::fabs(s_t[i] - tt[i]));
+        std::printf("softassign (variant %d) max |error| %g\n", variants[v], err4);
+        if (!(err4 < 1e-4)) return 4;
+    }
+    emsa_set_softassign_variant(EMSA_SOFTASSIGN_MATERIALISED);
 
     // icp(): noiseless pair, every source point has its exact partner
     float i_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, i_t[3] = {0, 0, 0};

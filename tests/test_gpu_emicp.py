@@ -71,6 +71,36 @@ def test_step_normalisers_match_oracle(E, oracle, sigma_p2):
     np.testing.assert_allclose(g["t"], o["t"], atol=2e-6)
 
 
+@pytest.mark.parametrize("k", [0, 13, 24, 30, 37])
+def test_step_normalisers_partial_overlap(E, oracle, k):
+    """A partial-overlap range-scan pair with few targets (60 x 1500): most
+    rows' C_r are made of a handful of distant terms and, from step 25 on,
+    hundreds of rows are floored. Step k of the sigma^2 = 0.01 * 0.9^k
+    schedule from the oracle's own state after k steps (the same float32
+    (R, t) to both sides): C_r (away from the floor), lambda_r and Lambda
+    within 1e-5 relative."""
+    from cuda_emicp_softassign_amd.clouds import range_scans
+    X, Y, _, _ = range_scans(60, 1500, seed=100)
+    Rt0 = E.multistart_rt0(X, Y, [np.eye(3)])[0].astype(np.float64)
+    sig = oracle.sigma_schedule(0.01, 2e-4, 0.9)
+    R, t = Rt0[:9].reshape(3, 3), Rt0[9:]
+    if k > 0:
+        R, t, _ = oracle.emicp(X.astype(np.float64), Y.astype(np.float64), R, t, sigma_inf=2e-4,
+                               max_iter=k)
+    R32, t32 = _f32(R), _f32(t)
+    o = oracle.emicp_step(X.astype(np.float64), Y.astype(np.float64), R32.astype(float),
+                          t32.astype(float), float(sig[k]), 0.01)
+    g = E.emicp_step(X, Y, R32.copy(), t32.copy(), float(sig[k]), 0.01)
+    keep = o["C"] > 2e-6
+    relC = np.max(np.abs(g["C"][keep] - o["C"][keep]) / o["C"][keep])
+    relL = np.max(np.abs(g["lam"] - o["lam"]) / np.maximum(o["lam"], 1e-300))
+    print("step %d sigma^2 %.3g: rel err C %.1e lambda %.1e, floored %d" %
+          (k, sig[k], relC, relL, (o["C"] <= 1e-6).sum()), flush=True)
+    assert relC <= 1e-5 and relL <= 1e-5
+    assert abs(g["record"][1] - o["Lambda"]) <= 1e-5 * o["Lambda"]
+    assert geodesic(g["R"].astype(np.float64), o["R"]) < 2e-5
+
+
 def test_floored_rows(E, oracle):
     X, Y, _, _ = uniform_cube(500, seed=3)
     Y = Y.copy()

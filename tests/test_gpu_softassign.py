@@ -112,3 +112,103 @@ def test_invalid_and_empty_schedules(E):
     R0 = rotation([0, 0, 1], 0.4).astype(np.float32)
     R, t = E.softassign(X, Y, R0.copy(), ht.copy(), E.registrationParameters(JMAX=0))
     assert np.array_equal(R, R0) and not t.any()
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def _steps_vs_oracle(E, oracle, X, Y, variant, temps, alpha=3.0, I1=3, R0=None, t0=None,
+                     moutlier=None):
+    """Inner iterations of emsa_softassign_step, each compared with the FP64
+    oracle's inner iteration started from the same state (R, t, o_row,
+    o_col): every sweep's row sums a_r and column sums b_c, the post-Sinkhorn
+    row sums s_r and Sigma within 1e-5 relative; Xc, Yc and t within 1e-5 of
+    the cloud extent; S within 1e-5 of its largest entry; R within 2e-5 rad.
+    Returns the largest relative normaliser error seen."""
+    mo = float(E.registrationParameters().moutlier) if moutlier is None else moutlier
+    orow = np.full(len(X), mo, np.float32)
+    ocol = np.full(len(Y), mo, np.float32)
+    R = np.eye(3) if R0 is None else np.asarray(R0, np.float64)
+    t = np.zeros(3) if t0 is None else np.asarray(t0, np.float64)
+    ext = np.ptp(X, axis=0).max()
+    worst = 0.0
+    for k, T in enumerate(temps):
+        o64r, o64c = orow.astype(np.float64), ocol.astype(np.float64)
+        o = oracle.softassign_inner(X.astype(np.float64), Y.astype(np.float64),
+                                    R.astype(np.float32).astype(np.float64),
+                                    t.astype(np.float32).astype(np.float64), float(T), alpha,
+                                    I1, o64r, o64c)
+        g = E.softassign_step(X, Y, R.astype(np.float32), t.astype(np.float32), T, alpha, I1,
+                              orow, ocol, variant=variant)
+        errs = [_rel(g["a"], o["a"]), _rel(g["b"], o["b"]), _rel(g["s"], o["rowsum"]),
+                abs(g["record"][1] - o["sumM"]) / abs(o["sumM"]),
+                _rel(orow, o64r), _rel(ocol, o64c)]
+        print("  %s step %d T=%g: rel err a %.1e b %.1e s %.1e Sigma %.1e o_row %.1e o_col %.1e"
+              % (variant, k, T, *errs), flush=True)
+        worst = max(worst, max(errs))
+        assert max(errs) <= 1e-5, (k, errs)
+        rec = g["record"]
+        np.testing.assert_allclose(rec[2:5], o["Xc"], atol=1e-5 * ext)
+        np.testing.assert_allclose(rec[5:8], o["Yc"], atol=1e-5 * ext)
+        np.testing.assert_allclose(rec[8:17], o["S"], atol=1e-5 * np.abs(o["S"]).max())
+        assert geodesic(g["R"].astype(np.float64), o["R"]) < 2e-5, k
+        np.testing.assert_allclose(g["t"], o["t"], atol=1e-5 * ext)
+        R, t = g["R"].astype(np.float64), g["t"].astype(np.float64)
+    return worst
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_step_normalisers_small(E, oracle, variant):
+    """Per sweep and inner iteration on a 300 x 280 case: a hot step, a
+    middle one and cold ones (slack state carried from step to step)."""
+    X, Y = _case()
+    _steps_vs_oracle(E, oracle, X, Y, variant, [100.0, 5.0, 0.5, 0.1, 0.05], alpha=0.05)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_step_normalisers_config1_size(E, oracle, variant):
+    """N = M = 5032 (surface + 10% outliers, reference defaults): the first 5
+    inner iterations of the schedule (T = 100) and 3 at the schedule's cold
+    end (T = 0.7, 0.66, 0.6232)."""
+    X, Y, _, _ = surface_with_outliers(5032, seed=1)
+    print("config-1 size, %s" % variant, flush=True)
+    _steps_vs_oracle(E, oracle, X, Y, variant, [100.0] * 5 + [0.7, 0.66, 0.6232])
+
+
+def test_step_rejects_bad_arguments(E):
+    X, Y = _case(n=50, m=40)
+    orow = np.zeros(50, np.float32)
+    ocol = np.zeros(40, np.float32)
+    with pytest.raises(E.EmsaError) as ei:
+        E.softassign_step(X, Y, np.eye(3), np.zeros(3), 1.0, 0.0, 0, orow, ocol)
+    assert ei.value.code == E._lib.EMSA_EINVAL
+    with pytest.raises(E.EmsaError) as ei:
+        E.softassign_step(X, Y, np.eye(3), np.zeros(3), 0.0, 0.0, 3, orow, ocol)
+    assert ei.value.code == E._lib.EMSA_ENONFINITE
+    with pytest.raises(ValueError):
+        E.softassign_step(X, Y, np.eye(3), np.zeros(3), 1.0, 0.0, 3, orow[:10], ocol)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_known_answer_fixture_transform(E, pq_pairs, variant):
+    """The reference's data/P1007.txt / data/Qnew1007.txt from 10 degrees off,
+    no slack, Sinkhorn run to convergence (I1 = 30), annealed to T < 1e-7:
+    the soft correspondences become the exact permutation and the result is
+    the least-squares transform of the true correspondences (the FP64 oracle
+    reaches it to 1e-11 rad, tests/test_oracle.py); in FP32 to 1e-5 rad."""
+    from tests.test_oracle import kabsch
+    P, Q = pq_pairs[1007]
+    Rs, ts = kabsch(P, Q)
+    R0 = Rs @ rotation([1, 2, 3], np.deg2rad(10))
+    t0 = P.mean(0) - R0 @ Q.mean(0)
+    p = E.registrationParameters(JMAX=56, I0=2, I1=30, alpha=0.0, T_0=0.01, TFACTOR=0.8,
+                                 moutlier=0.0)
+    R, t = E.softassign(P.astype(np.float32), Q.astype(np.float32), R0.astype(np.float32),
+                        t0.astype(np.float32), p, variant=variant)
+    err = geodesic(R.astype(np.float64), Rs)
+    print("  %s: rot err %.2e rad, |dt| %.2e" % (variant, err, np.abs(t - ts).max()), flush=True)
+    assert err < 1e-5
+    assert np.abs(t - ts).max() < 1e-5 * np.ptp(P, axis=0).max()

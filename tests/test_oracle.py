@@ -11,13 +11,21 @@ so that a dropped term, a wrong sign or index, or a transposed operand fails:
                             eigensolver, and closed forms (uniform affinities
                             at huge sigma^2, floored rows);
   * EM-ICP loop          -> exact recovery of the P/Qnew transform, FP32 vs FP64;
-  * Softassign           -> the literal loops, Sinkhorn invariants, FP32 vs FP64.
+  * Softassign           -> the literal loops (every sweep's row and column
+                            sums included), Sinkhorn invariants, FP32 vs FP64,
+                            exact recovery of the P/Qnew transform (no slack,
+                            converged Sinkhorn, annealed below the spacing);
+  * batched fixture      -> tests/golden/config3_oracle.npz re-derived from
+                            the oracle on its smallest instance.
 """
+import os
+
 import numpy as np
 import pytest
 
 from cuda_emicp_softassign_amd.clouds import geodesic, rotation, uniform_cube
 from tests import _literal
+from tests.conftest import GOLDEN
 
 
 def eig_lapack(N):
@@ -225,6 +233,9 @@ def test_softassign_inner_matches_literal_loops(oracle, T):
         o = oracle.softassign_inner(X, Y, R0, t0, T, 3.0, 3, orow, ocol)
         np.testing.assert_allclose(o["rowsum"], lit["rowsum"], rtol=1e-12)
         np.testing.assert_allclose(o["colsum"], lit["colsum"], rtol=1e-12)
+        # every sweep's row and column sums before their division
+        np.testing.assert_allclose(o["a"], lit["a"], rtol=1e-12)
+        np.testing.assert_allclose(o["b"], lit["b"], rtol=1e-12)
         np.testing.assert_allclose(orow, orow_l, rtol=1e-12)
         np.testing.assert_allclose(ocol, ocol_l, rtol=1e-12)
         np.testing.assert_allclose(o["S"], lit["S"], atol=1e-12)
@@ -262,3 +273,45 @@ def test_softassign_f32_tracks_f64(oracle):
     d = [geodesic(tr[i, 17:26].reshape(3, 3), tr32[i, 17:26].reshape(3, 3))
          for i in range(len(tr))]
     assert max(d) < 1e-5
+
+
+def test_softassign_recovers_fixture_transform(oracle, pq_pairs):
+    """Known answer: the reference's data/P1007.txt and data/Qnew1007.txt
+    (row correspondence, one exact rigid transform) from a start 10 degrees
+    off. With no slack (moutlier 0) and Sinkhorn run to convergence (I1 = 30),
+    annealed until even the closest non-partners no longer count (T < 1e-7
+    against squared spacings >= 5e-7), the soft correspondences become the
+    exact permutation and Softassign must return the least-squares (Kabsch)
+    transform of the true correspondences. A wrong sign, index or transpose
+    in the affinities, the Sinkhorn sweeps, the weighted centroids, S or
+    Horn's step would not converge there."""
+    P, Q = pq_pairs[1007]
+    Rs, ts = kabsch(P, Q)  # P ~ Rs Q + ts
+    R0 = Rs @ rotation([1, 2, 3], np.deg2rad(10))
+    t0 = P.mean(0) - R0 @ Q.mean(0)
+    R, t, tr = oracle.softassign(P, Q, R0, t0, JMAX=56, I0=2, I1=30, alpha=0.0, T_0=0.01,
+                                 TFACTOR=0.8, moutlier=0.0)
+    assert tr[-1, 0] < 1e-7
+    assert geodesic(R, Rs) < 1e-9
+    np.testing.assert_allclose(t, ts, atol=1e-10)
+
+
+def test_config3_fixture_reproduces(oracle):
+    """tests/golden/config3_oracle.npz holds what the oracle computes: the
+    smallest instance's clouds regenerate bit for bit (SHA-256) and two of
+    its starts re-run to the stored FP64 and FP32 results."""
+    from tests.golden import make_config3_oracle as G
+    d = np.load(os.path.join(GOLDEN, "config3_oracle.npz"), allow_pickle=False)
+    i = int(np.argmin(d["N"] * d["M"]))
+    X, Y = G.instance(int(d["instances"][i]))
+    assert G.clouds_sha(X, Y) == str(d["sha"][i])
+    assert (len(X), len(Y)) == (d["N"][i], d["M"][i])
+    for s in (0, 13):
+        rt0 = d["rt0"][i, s]
+        R, t, _ = oracle.emicp(X.astype(np.float64), Y.astype(np.float64),
+                               rt0[:9].astype(np.float64).reshape(3, 3), rt0[9:].astype(np.float64))
+        np.testing.assert_allclose(np.concatenate([R.reshape(9), t]), d["oracle_f64"][i, s],
+                                   rtol=0, atol=1e-12)
+        R, t, _ = oracle.emicp(X, Y, rt0[:9].reshape(3, 3), rt0[9:], precision="f32")
+        np.testing.assert_array_equal(np.concatenate([R.reshape(9), t]).astype(np.float64),
+                                      d["restatement_f32"][i, s])
