@@ -26,7 +26,7 @@ from ._lib import (EMSA_OK, SOFTASSIGN_LOGDOMAIN, SOFTASSIGN_MATERIALISED, TRACE
 
 __all__ = [
     "registrationParameters", "emicp", "softassign", "findRTfromS", "cloud2data", "printRT",
-    "emicp_traced", "emicp_step", "softassign_traced", "softassign_step", "emicp_iterations",
+    "emicp_traced", "emicp_step", "emicp_cpu", "softassign_traced", "softassign_step", "emicp_iterations",
     "emicp_dev",
     "EmsaError", "SOFTASSIGN_MATERIALISED", "SOFTASSIGN_LOGDOMAIN", "TRACE_W",
     "emicp_batched", "emicp_batched_dev", "autotune_emicp", "multistart_rt0", "select_best",
@@ -145,6 +145,26 @@ def emicp_traced(cloud_target, cloud_source, h_R, h_t, param=None):
         _fptr(X), N, _fptr(Y), M, _fptr(R), _fptr(t), C.byref(p), _dptr(tr), n))
     Rr, tt = _writeback(h_R, h_t, R, t)
     return Rr, tt, tr[:rc]
+
+
+def emicp_cpu(cloud_target, cloud_source, h_R, h_t, param=None, traced=False):
+    """EM-ICP on the host CPU in FP32 (src/emicp_cpu.cpp:186-547): the
+    reference's iteration and operation order, no GPU involved. Returns (R, t)
+    (and the per-step trace when traced) and writes R, t into h_R, h_t when
+    those are numpy arrays."""
+    param = param or registrationParameters()
+    X, N = cloud2data(cloud_target)
+    Y, M = cloud2data(cloud_source)
+    R, t = _inout(h_R, h_t)
+    p = param.to_c()
+    L = _lib.load()
+    n = max(_lib.check("emicp_iterations", L.emsa_emicp_iterations(C.byref(p))), 1)
+    tr = np.zeros((n, TRACE_W))
+    rc = _lib.check("emicp_cpu", L.emsa_emicp_cpu(
+        _fptr(X), N, _fptr(Y), M, _fptr(R), _fptr(t), C.byref(p),
+        _dptr(tr) if traced else None, n if traced else 0))
+    Rr, tt = _writeback(h_R, h_t, R, t)
+    return (Rr, tt, tr[:rc]) if traced else (Rr, tt)
 
 
 def emicp_step(cloud_target, cloud_source, R, t, sigma_p2, d_02=0.01):

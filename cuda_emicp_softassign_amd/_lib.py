@@ -59,6 +59,7 @@ SIGNATURES = {
     "emsa_emicp_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, _dp, C.c_int]),
     "emsa_emicp_step": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float, C.c_float,
                                   _dp, _dp, _dp]),
+    "emsa_emicp_cpu": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, _dp, C.c_int]),
     "emsa_softassign": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int]),
     "emsa_softassign_traced": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp, C.c_int,
                                          _dp, C.c_int]),

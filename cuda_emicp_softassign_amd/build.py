@@ -13,7 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemsa.so")
 SOURCES = ["capi.hip", "emicp.hip", "softassign.hip", "batched.hip", "adjoint.hip",
-           "sa_unrolled.hip", "icp.hip", "probe.hip", "noalloc.hip", "registration.cpp"]
+           "sa_unrolled.hip", "icp.hip", "probe.hip", "noalloc.hip", "emicp_cpu.cpp",
+           "registration.cpp"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

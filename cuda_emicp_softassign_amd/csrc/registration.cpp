@@ -61,6 +61,16 @@ void emicp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
                                h_t, &p));
 }
 
+void emicp_cpu(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
+               const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
+               const registrationParameters& param) {
+    const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
+    const emsa_params p = to_params(param);
+    const int rc = emsa_emicp_cpu(X.data(), (int)(X.size() / 3), Y.data(), (int)(Y.size() / 3),
+                                  h_R, h_t, &p, nullptr, 0);
+    report("emicp_cpu", rc < 0 ? rc : EMSA_OK);
+}
+
 void icp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
          const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
          const registrationParameters& param) {

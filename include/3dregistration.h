@@ -34,6 +34,21 @@
 #endif
 
 #ifndef EMSA_HAVE_PCL
+namespace boost {
+// The members of boost::shared_array that cloud2data's second overload uses.
+template <class T>
+class shared_array {
+    std::shared_ptr<T> p_;
+
+public:
+    shared_array() = default;
+    explicit shared_array(T* p) : p_(p, std::default_delete<T[]>()) {}
+    void reset(T* p = nullptr) { p_.reset(p, std::default_delete<T[]>()); }
+    T* get() const { return p_.get(); }
+    T& operator[](std::ptrdiff_t i) const { return p_.get()[i]; }
+    explicit operator bool() const { return (bool)p_; }
+};
+}  // namespace boost
 namespace pcl {
 struct PointXYZ {
     float x, y, z;
@@ -101,6 +116,10 @@ void softassign(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
 void emicp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
            const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
            const registrationParameters& param);
+// EM-ICP on the host CPU in FP32 (src/emicp_cpu.cpp:186-547).
+void emicp_cpu(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
+               const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_source, float* h_R, float* h_t,
+               const registrationParameters& param);
 // ICP with exact nearest-neighbour correspondences (src/icp.cpp:122-289,
 // declared at src/3dregistration.h:96-99): param.maxIteration iterations.
 void icp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
@@ -112,6 +131,20 @@ void findRTfromS(const float* h_Xc, const float* h_Yc, const float* h_S, float* 
 void printRT(const float* R, const float* t);
 
 void cloud2data(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud, float** X, int& Xsize);
+
+// src/cloud2data.cpp:50-67: the SoA copy owned by a shared_array. Inline, so
+// that it works with Boost's shared_array (PCL builds) as with the one above.
+inline void cloud2data(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud,
+                       boost::shared_array<float>& X, int& Xsize) {
+    Xsize = cloud ? (int)cloud->size() : 0;
+    float* h = new float[Xsize > 0 ? 3 * (std::size_t)Xsize : 1];
+    for (int i = 0; i < Xsize; ++i) {
+        h[i] = cloud->points[i].x;
+        h[Xsize + i] = cloud->points[i].y;
+        h[2 * Xsize + i] = cloud->points[i].z;
+    }
+    X.reset(h);
+}
 
 #define CUDA_SAFE_CALL(a) a
 #define CUT_SAFE_CALL(a) a

@@ -91,6 +91,17 @@ int emsa_emicp_traced(const float *X, int N, const float *Y, int M, float *R, fl
 int emsa_emicp_step(const float *X, int N, const float *Y, int M, float *R, float *t,
                     float sigma_p2, float d_02, double *C, double *lambda, double *record);
 
+/* EM-ICP on the host CPU in FP32 (no GPU needed). Replaces emicp_cpu()
+ * (src/emicp_cpu.cpp:186-547; declared at src/3dregistration.h:91-95): the
+ * same iteration and FP32 operation order, libm expf/sqrtf, source rows split
+ * over host threads (EMSA_CPU_THREADS or OMP_NUM_THREADS, else all cores;
+ * results do not depend on the count), Horn's step in FP64. trace (nullable,
+ * trace_cap records): one EMSA_TRACE_W record per sigma^2 step as in
+ * emsa_emicp_traced. Returns the number of sigma^2 steps run (records written
+ * when trace is given) or a negative code. */
+int emsa_emicp_cpu(const float *X, int N, const float *Y, int M, float *R, float *t,
+                   const emsa_params *p, double *trace, int trace_cap);
+
 /* Softassign on the GPU. Replaces softassign() (src/softassign.cu:256-886;
  * declared at src/3dregistration.h:82-86). variant: EMSA_SOFTASSIGN_*. */
 int emsa_softassign(const float *X, int N, const float *Y, int M, float *R, float *t,

@@ -1,6 +1,7 @@
 // A caller written against the reference API (src/3dregistration.h): builds
-// two clouds, runs emicp(), softassign() and icp() through the drop-in header and
-// checks that the known rigid transform comes back. Exit status 0 on success.
+// two clouds, runs emicp(), emicp_cpu(), softassign() and icp() through the
+// drop-in header and checks that the known rigid transform comes back. Exit
+// status 0 on success.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -39,12 +40,26 @@ int main() {
     std::printf("emicp max |error| %g\n", err);
     if (!(err < 1e-4)) return 1;
 
+    // emicp_cpu(): the same registration on the host CPU in FP32
+    float c_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, c_t[3] = {0, 0, 0};
+    emicp_cpu(target, source, c_R, c_t, param);
+    double errc = 0;
+    for (int i = 0; i < 9; ++i) errc = std::fmax(errc, std::fabs(c_R[i] - Rt[i]));
+    for (int i = 0; i < 3; ++i) errc = std::fmax(errc, std::fabs(c_t[i] - tt[i]));
+    std::printf("emicp_cpu max |error| %g\n", errc);
+    if (!(errc < 1e-4)) return 6;
+
     float* X = nullptr;
     int Xsize = 0;
     cloud2data(target, &X, Xsize);
     const bool layout_ok = Xsize == 800 && X[0] == target->points[0].x && X[800] == target->points[0].y;
     delete[] X;
     if (!layout_ok) return 2;
+    boost::shared_array<float> Xs;
+    int Xs_size = 0;
+    cloud2data(source, Xs, Xs_size);
+    if (!(Xs_size == 800 && Xs[1] == source->points[1].x && Xs[1600 + 1] == source->points[1].z))
+        return 2;
 
     // findRTfromS on the exact correspondence
     float Xc[3] = {0, 0, 0}, Yc[3] = {0, 0, 0}, S[9] = {0};

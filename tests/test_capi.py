@@ -43,6 +43,7 @@ def test_dropin_cxx_symbols_exported():
                           os.path.join(ROOT, "cuda_emicp_softassign_amd", "libemsa.so")],
                          capture_output=True, text=True, check=True).stdout
     for sig in ["emicp(std::shared_ptr<pcl::PointCloud<pcl::PointXYZ> >",
+                "emicp_cpu(std::shared_ptr<pcl::PointCloud<pcl::PointXYZ> >",
                 "softassign(std::shared_ptr<pcl::PointCloud<pcl::PointXYZ> >",
                 "findRTfromS(float const*, float const*, float const*, float*, float*)",
                 "printRT(float const*, float const*)", "cloud2data("]:
