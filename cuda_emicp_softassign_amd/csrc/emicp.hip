@@ -748,6 +748,13 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
     float (*st)[kMfPiece] = stage[wave];
     const int j = lane & 3;  // this lane's component in the matrix product
+    // the operand's constant fourth component (B = sum e * 1): the compensated
+    // instance writes it once (restaging it per piece kept a constant float4
+    // live across the loop and spilled it at 8 waves per SIMD); the plain one
+    // restages it with each piece (same-process A/B: writing it once changed
+    // its register allocation and made its launches 7% slower)
+    static_assert(kMfPiece == 4 * kWave, "one float4 of ones per lane");
+    if (COMP) *reinterpret_cast<float4*>(&st[3][4 * lane]) = make_float4(1.f, 1.f, 1.f, 1.f);
     float C = 0.f;
     f4 D[NACC];
 #pragma unroll
@@ -793,7 +800,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                     *reinterpret_cast<const float4*>(d.Xpad + ldx + col);
                 *reinterpret_cast<float4*>(&st[2][sc]) =
                     *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + col);
-                *reinterpret_cast<float4*>(&st[3][sc]) = make_float4(1.f, 1.f, 1.f, 1.f);
+                if (!COMP) *reinterpret_cast<float4*>(&st[3][sc]) = make_float4(1.f, 1.f, 1.f, 1.f);
             }
         }
         wave_lds_sync();
