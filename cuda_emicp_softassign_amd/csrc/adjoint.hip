@@ -21,10 +21,10 @@
 //     shared by 256 pairs.
 //  3. adj_rows: per row, dL/dy_r = R^T dL/dp_r + the direct moment terms, and
 //     the registration partials dL/dR_k = sum dp y^T, dL/dt_k = sum dp.
-//  4. adj_finish: per registration, the new (dL/dR_k, dL/dt_k) and
+//  4. adj_cols: per target column, dL/dx_c summed over row tiles in a fixed
+//     order (+ the floored rows' 1/N share); its column-0 thread of each
+//     registration also writes the new (dL/dR_k, dL/dt_k) and
 //     dL/dsigma_0^2 += f^k dL/dsigma_k^2.
-//  5. adj_cols: per target column, dL/dx_c summed over row tiles in a fixed
-//     order (+ the floored rows' 1/N share).
 // All reductions have a fixed order; no workgroup waits on another.
 #include "emsa_device.h"
 #include "emsa_internal.h"
@@ -311,13 +311,18 @@ __device__ inline RowCoef row_coef(const AdjDesc& d, const double* coef, int k, 
 }
 
 // Stage 2: a workgroup = 256 source rows x one column chunk; lanes hold 4
-// rows each, the 4 waves split the chunk. Per pair: 22 VALU + 1 exp (25 with
-// the compensated difference COMP, used at small sigma) and ~1 more for the
+// rows each, the 4 waves split the chunk. The differences are taken in units
+// of the exponent scale, as the forward takes them: d' = ka x - ka p with
+// ka = sqrt(log2 e / (2 sigma^2)), so e = exp2(-|d'|^2) needs no scaling
+// multiply; the sums of h d' and h |d'|^2 are ka and ka^2 times those of
+// h d and h |d|^2 (the rows and columns kernels divide in FP64), and the
+// columns' e gW terms take ka gW. Per pair: 21 VALU + 1 exp (24 with the
+// compensated difference COMP, used at small sigma) and ~1 more for the
 // column sums. The row coefficients are pre-scaled by -1/sigma^2 so that
 // h = ge e needs one multiply.
 template <bool COMP>
 __global__ void __launch_bounds__(kWave * kAdjWaves)
-adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk, float minv_s2) {
+adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float ka, float minv_s2) {
     const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.tileBase; });
     const AdjDesc& d = regs[ri];
     const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
@@ -344,9 +349,10 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
             yy = d.Y[(size_t)M + r];
             yz = d.Y[2 * (size_t)M + r];
         }
-        const double pxd = Rk[0] * yx + Rk[1] * yy + Rk[2] * yz + tk[0];
-        const double pyd = Rk[3] * yx + Rk[4] * yy + Rk[5] * yz + tk[1];
-        const double pzd = Rk[6] * yx + Rk[7] * yy + Rk[8] * yz + tk[2];
+        const double kad = (double)ka;  // the row in units of the exponent scale
+        const double pxd = kad * (Rk[0] * yx + Rk[1] * yy + Rk[2] * yz + tk[0]);
+        const double pyd = kad * (Rk[3] * yx + Rk[4] * yy + Rk[5] * yz + tk[1]);
+        const double pzd = kad * (Rk[6] * yx + Rk[7] * yy + Rk[8] * yz + tk[2]);
         px[j] = (float)pxd;
         py[j] = (float)pyd;
         pz[j] = (float)pzd;
@@ -363,9 +369,9 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
             mWx[j] = (float)(m * o.gW[0]);
             mWy[j] = (float)(m * o.gW[1]);
             mWz[j] = (float)(m * o.gW[2]);
-            gWx[j] = (float)o.gW[0];
-            gWy[j] = (float)o.gW[1];
-            gWz[j] = (float)o.gW[2];
+            gWx[j] = (float)(o.gW[0] * kad);
+            gWy[j] = (float)(o.gW[1] * kad);
+            gWz[j] = (float)(o.gW[2] * kad);
         }
     }
 
@@ -391,7 +397,8 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
             float cx = 0.f, cy = 0.f, cz = 0.f;
 #pragma unroll
             for (int j = 0; j < kAdjRPL; ++j) {
-                float dx = xx - px[j], dy = xy - py[j], dz = xz - pz[j];
+                float dx = fmaf(xx, ka, -px[j]), dy = fmaf(xy, ka, -py[j]),
+                      dz = fmaf(xz, ka, -pz[j]);
                 if (COMP) {
                     dx -= lx[j];
                     dy -= ly[j];
@@ -400,7 +407,7 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
                 float s = dx * dx;
                 s = fmaf(dy, dy, s);
                 s = fmaf(dz, dz, s);
-                const float e = __builtin_amdgcn_exp2f(s * negk);
+                const float e = __builtin_amdgcn_exp2f(-s);
                 // h = -(dB + 2 e dC + dW . x) e / sigma^2
                 const float mg = fmaf(mWz[j], xz, fmaf(mWy[j], xy, fmaf(mWx[j], xx, fmaf(e, mC2[j], mB[j]))));
                 const float h = mg * e;
@@ -456,7 +463,7 @@ adj_pairs_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, float negk,
 // Stage 3: one workgroup per 256-row tile, one thread per row.
 __global__ void __launch_bounds__(kAdjRows)
 adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_scale,
-                double outlier_dsig) {
+                double outlier_dsig, double inv_ka) {
     const int ri = find_by(regs, nregs, blockIdx.x, [](const AdjDesc& a) { return a.rowBase; });
     const AdjDesc& d = regs[ri];
     const int M = d.M, chunks = d.chunks;
@@ -475,10 +482,11 @@ adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_
             gp[2] += q[2 * kAdjRows];
             gsum += q[3 * kAdjRows];
         }
-        // accumulated +h d; dL/dp = -sum h d
-        gp[0] = -gp[0];
-        gp[1] = -gp[1];
-        gp[2] = -gp[2];
+        // accumulated +h d' = ka h d; dL/dp = -sum h d
+        gp[0] = -gp[0] * inv_ka;
+        gp[1] = -gp[1] * inv_ka;
+        gp[2] = -gp[2] * inv_ka;
+        gsum *= inv_ka * inv_ka;  // sum h |d'|^2 = ka^2 sum h |d|^2
         const float yx = d.Y[r], yy = d.Y[(size_t)M + r], yz = d.Y[2 * (size_t)M + r];
         const RowCoef o = row_coef(d, d.coef, k, r, yx, yy, yz);
         double Rk[9], tk[3];
@@ -509,39 +517,44 @@ adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_
     if (threadIdx.x < kAdjMomW) d.rtmom[(size_t)rt * kAdjMomW + threadIdx.x] = tot;
 }
 
-// Stage 4: one workgroup per registration.
-__global__ void __launch_bounds__(64)
-adj_finish_kernel(const AdjDesc* __restrict__ regs, int k, double fk) {
-    const AdjDesc& d = regs[blockIdx.x];
-    const int i = threadIdx.x;
-    if (i >= kAdjMomW) return;
-    // a failed Horn solve left (R, t) unchanged: its gradient passes through
-    const bool failed = d.rec[(size_t)k * EMSA_TRACE_W + 31] != 0.0;
-    double s = 0.0;
-    for (int t = 0; t < d.rowTiles; ++t) s += d.rtmom[(size_t)t * kAdjMomW + i];
-    if (i < 12) {
-        if (!failed) d.adj[i] = s;              // dL/dR_k, dL/dt_k
-    }
-    else if (i == 12) d.adj[12] += fk * s;      // dL/dsigma_0^2
-    else d.adj[i] = s;                          // 13..15: floored rows' dL/dV sum
-}
-
-// Stage 5: one thread per target column.
+// Stage 4: one thread per target column, which also finishes the step for
+// its registration: the row tiles' moments summed in tile order give the
+// floored rows' dL/dV sum (every column thread sums these 3 itself) and the
+// next step's dL/dR, dL/dt and the dL/dsigma_0^2 term (written by the
+// registration's column-0 thread: no other thread reads the state here).
+// fk = factor^k. A step whose Horn solve failed in the forward left (R, t)
+// unchanged, so its gradient passes through.
 __global__ void __launch_bounds__(256)
-adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols) {
+adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols, int k,
+                double fk, double inv_ka) {
     const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gc >= totalCols) return;
     const int ri = find_by(regs, nregs, (int)gc, [](const AdjDesc& a) { return a.colBase; });
     const AdjDesc& d = regs[ri];
     const int c = (int)gc - d.colBase;
     if (c >= d.N) return;
-    double g[3] = {d.floorA * d.adj[13], d.floorA * d.adj[14], d.floorA * d.adj[15]};
+    double gfl[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < d.rowTiles; ++t)
+        for (int i = 0; i < 3; ++i) gfl[i] += d.rtmom[(size_t)t * kAdjMomW + 13 + i];
+    if (c == 0) {
+        const bool failed = d.rec[(size_t)k * EMSA_TRACE_W + 31] != 0.0;
+        double m[13];
+        for (int i = 0; i < 13; ++i) m[i] = 0.0;
+        for (int t = 0; t < d.rowTiles; ++t)
+            for (int i = 0; i < 13; ++i) m[i] += d.rtmom[(size_t)t * kAdjMomW + i];
+        if (!failed)
+            for (int i = 0; i < 12; ++i) d.adj[i] = m[i];  // dL/dR_k, dL/dt_k
+        d.adj[12] += fk * m[12];                          // dL/dsigma_0^2
+        for (int i = 0; i < 3; ++i) d.adj[13 + i] = gfl[i];
+    }
+    double g[3] = {0.0, 0.0, 0.0};  // the pair terms, ka times dL/dx (see adj_pairs_kernel)
     for (int t = 0; t < d.rowTiles; ++t) {
         const float* o = d.colpart + ((size_t)t * d.ldx + c) * 3;
         g[0] += o[0];
         g[1] += o[1];
         g[2] += o[2];
     }
+    for (int i = 0; i < 3; ++i) g[i] = g[i] * inv_ka + d.floorA * gfl[i];
     d.gX[c] += (float)g[0];
     d.gX[(size_t)d.N + c] += (float)g[1];
     d.gX[2 * (size_t)d.N + c] += (float)g[2];
@@ -725,7 +738,8 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
     const std::vector<float> sig = unrolled_sigmas(sigma2_0, factor, steps);
     for (int k = steps - 1; k >= 0; --k) {
         const double s2 = (double)sig[k];
-        const float negk = (float)(-kLog2e / (2.0 * s2));
+        const float ka = (float)std::sqrt(kLog2e / (2.0 * s2));  // as the forward's
+        const double inv_ka = 1.0 / (double)ka;
         const float minv = (float)(-1.0 / s2);
         const double out = (double)expf(-d_02 / sig[k]);
         const double outlier_dsig = out * (double)d_02 / (s2 * s2);  // d exp(-d02/s2) / d s2
@@ -736,15 +750,14 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
         hipLaunchKernelGGL(adj_solve_kernel, dim3(count), dim3(64), 0, s, desc, k);
         if (sig[k] < kCompSigma2)
             hipLaunchKernelGGL(adj_pairs_kernel<true>, dim3((unsigned)tiles),
-                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, negk, minv);
+                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, ka, minv);
         else
             hipLaunchKernelGGL(adj_pairs_kernel<false>, dim3((unsigned)tiles),
-                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, negk, minv);
+                               dim3(kWave * kAdjWaves), 0, s, desc, count, k, ka, minv);
         hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)rowTiles), dim3(kAdjRows), 0, s, desc,
-                           count, k, gsig_scale, outlier_dsig);
-        hipLaunchKernelGGL(adj_finish_kernel, dim3(count), dim3(64), 0, s, desc, k, fk);
+                           count, k, gsig_scale, outlier_dsig, inv_ka);
         hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
-                           desc, count, cols);
+                           desc, count, cols, k, fk, inv_ka);
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(adj_output_kernel, dim3(count), dim3(64), 0, s, desc, dgRt0, dgsigma);
