@@ -141,10 +141,12 @@ def cull_sweep(_lib, torch):
 def counters_mode(which, E, _lib, torch):
     """Workloads for rocprofv3 --pmc runs (no hipGraph in the process):
     affinity -- the 66-step batched EM-ICP sequence of the 8 x 10k workload,
-    launched directly, twice (emsa_probe_emicp_sequence); softassign -- one
-    materialised N = M = 5032
-    registration at the reference defaults. Then a 256 MiB device copy of known
-    size for the counter scale."""
+    launched directly, twice (emsa_probe_emicp_sequence); softassign /
+    softassign-log -- one N = M = 5032 registration at the reference defaults,
+    materialised / log-domain; softassign-20k -- materialised N = M = 20,000
+    (K is 1.6 GB, larger than every cache), one warm-up call, then 10 inner
+    iterations (JMAX = 2, I0 = 5). Then a 256 MiB device copy of known size for
+    the counter scale."""
     from cuda_emicp_softassign_amd.clouds import surface_with_outliers, uniform_cube
     dev = torch.device("cuda:0")
     out = {"counters": which}
@@ -164,12 +166,24 @@ def counters_mode(which, E, _lib, torch):
             ctypes.byref(seq_n), None, 0))
         out.update(avg_launch_us=seq_us.value / seq_n.value,
                    pairs_per_launch=seq_pairs.value / seq_n.value, launches=seq_n.value)
-    else:
+    elif which in ("softassign", "softassign-log"):
         X, Y, _, _ = surface_with_outliers(5032, seed=1)
         p = E.registrationParameters()
+        variant = "logdomain" if which == "softassign-log" else "materialised"
+        t0 = time.perf_counter()
+        E.softassign(X, Y, np.eye(3), np.zeros(3), p, variant=variant)
+        out["ms"] = (time.perf_counter() - t0) * 1e3
+    else:  # softassign-20k: materialised, N = M = 20,000 (the 1.6 GB K exceeds every cache)
+        X, Y, _, _ = surface_with_outliers(20000, seed=1)
+        p = E.registrationParameters(JMAX=2, I0=5)  # 10 inner iterations at T = 100, 95
+        E.softassign(X, Y, np.eye(3), np.zeros(3), E.registrationParameters(JMAX=1, I0=1))
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         E.softassign(X, Y, np.eye(3), np.zeros(3), p, variant="materialised")
-        out["ms"] = (time.perf_counter() - t0) * 1e3
+        ms = (time.perf_counter() - t0) * 1e3
+        n_in = p.JMAX * p.I0
+        out.update(ms=ms, inner_iterations=n_in, ms_per_inner_iteration=ms / n_in,
+                   k_bytes=4.0 * 20000 * 20224)
     src = torch.ones(256 * (1 << 18), device=dev)
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -385,7 +399,8 @@ def main():
     ap.add_argument("--cull-sweep", action="store_true",
                     help="only the EM-ICP affinity of the 8 x 10k batch at a range of sigma^2, "
                          "with and without skipping zero column groups")
-    ap.add_argument("--counters", choices=["affinity", "softassign"], default=None,
+    ap.add_argument("--counters", choices=["affinity", "softassign", "softassign-log",
+                                           "softassign-20k"], default=None,
                     help="counter-collection mode: run only this kernel's workload with direct "
                          "launches (no graph replay), then a 256 MiB device copy that "
                          "calibrates FETCH_SIZE / WRITE_SIZE; prints a short JSON line")

@@ -684,6 +684,9 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // with no such group is not even staged. visits (nullable, measurement): receives the
 // number of (valid row, valid column) pairs the launch evaluates.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
+// row stride of a wave's stage: 4 floats of padding put the four component
+// rows, which lanes 4b + j read together (row j), in different LDS banks
+constexpr int kMfStageLd = kMfPiece + 4;
 
 template <bool COMP, int UNR, int NACC, bool FUSE>
 __global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
@@ -745,8 +748,8 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         }
     }
 
-    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfPiece];
-    float (*st)[kMfPiece] = stage[wave];
+    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfStageLd];
+    float (*st)[kMfStageLd] = stage[wave];
     const int j = lane & 3;  // this lane's component in the matrix product
     // the operand's constant fourth component (B = sum e * 1): the compensated
     // instance writes it once (restaging it per piece kept a constant float4
@@ -854,7 +857,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     // (slots 2, 3, 4), j = 3 -> B (slot 1). Each wave's 5 x 64 partials go in
     // its own stage, which it no longer reads (its LDS accesses are in order),
     // so the workgroup needs 16 KB of LDS and 8 waves per SIMD fit.
-    static_assert(5 * kEmRows <= 4 * kMfPiece, "partials fit a wave's stage");
+    static_assert(5 * kEmRows <= 4 * kMfStageLd, "partials fit a wave's stage");
     float* red = &stage[wave][0][0];
     red[lane] = C;
     const int q = (j == 3) ? 1 : 2 + j;
@@ -866,9 +869,9 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         float acc[5];
 #pragma unroll
         for (int qq = 0; qq < 5; ++qq) {
-            acc[qq] = stage[0][0][qq * kEmRows + lane];
+            acc[qq] = (&stage[0][0][0])[qq * kEmRows + lane];
 #pragma unroll
-            for (int w = 1; w < kEmWaves; ++w) acc[qq] += stage[w][0][qq * kEmRows + lane];
+            for (int w = 1; w < kEmWaves; ++w) acc[qq] += (&stage[w][0][0])[qq * kEmRows + lane];
         }
         if (FUSE && d.chunks == 1) {
             // the tile spans the whole target (one chunk): these are the
