@@ -42,11 +42,10 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
 # (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
 # FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
-# WRITE_SIZE, run on `bench.py --counters affinity`: the plain-difference
-# instance fetches 15.6 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and
-# writes 8.4 MB of partial sums; the compensated one adds ~20 MB of register
-# spill traffic (sequence average over its 66 launches)
-AFFINITY_TRAFFIC_BYTES = 39.3e6
+# WRITE_SIZE, run on `bench.py --counters affinity`: both instances fetch
+# 15.6 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and write 8.4 MB of
+# partial sums; no register spills
+AFFINITY_TRAFFIC_BYTES = 24.0e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
@@ -576,6 +575,10 @@ def main():
             "pair_ceiling_per_s": ceiling,
             "pair_ceiling_serial_per_s": ceiling_serial,
             "frac_of_binding_ceiling": pair_rate / ceiling,
+            # the unfused count: 11 VALU instructions per pair (3 sub, mul + 2 FMA
+            # for |d|^2, scale, C, B and 3 W accumulations), or the exp rate
+            "pair_ceiling_11_valu_per_s": min(pe.value, pf.value / 11.0),
+            "frac_of_11_valu_ceiling": pair_rate / min(pe.value, pf.value / 11.0),
         },
     }
 
