@@ -380,6 +380,88 @@ def fwd_bwd_emicp(torch, dev, stream, timed, steps, B=32, n=4096, unrolled=20):
             "fwd_ms": f_ms, "fwd_bwd_ms": fb_ms, "ratio": fb_ms / f_ms}
 
 
+def _median_s(fn, runs=5):
+    """1 warm-up call, then the median wall-clock time of `runs` calls."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        c0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - c0)
+    return float(np.median(ts))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(O, X, Y, n, n_sigma, X0, Y0, gpu0, geodesic):
+    """The FP32 OpenMP restatements (oracle/) on the host cores, each timed as
+    1 warm-up + the median of 5 wall-clock runs: EM-ICP at N = M = 1007 (full
+    66-step registration, also the parity anchor against the FP64 oracle and
+    the GPU), at the headline size (the first 3 sigma^2 steps, extrapolated to
+    a registration and labelled so), and Softassign at N = M = 5032 (2 inner
+    iterations at T_0 = 100, extrapolated to the 500 of a registration; the
+    reference has no CPU Softassign, so this is not a reference path)."""
+    from cuda_emicp_softassign_amd.clouds import surface_with_outliers
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # config 0: full registration, and its parity anchor
+    c0_s = _median_s(lambda: O.emicp(X0, Y0, precision="f32"))
+    R32, t32, _ = O.emicp(X0, Y0, precision="f32")
+    R64, t64, _ = O.emicp(X0.astype(np.float64), Y0.astype(np.float64))
+    ext0 = float((X0.max(0) - X0.min(0)).max())
+    # headline size: the first k sigma^2 steps
+    k = 3
+    s10 = _median_s(lambda: O.emicp(X, Y, precision="f32", max_iter=k)) / k
+    # Softassign FP32 restatement, 2 inner iterations
+    Xs, Ys, _, _ = surface_with_outliers(5032, seed=1)
+    sa_s = _median_s(lambda: O.softassign(Xs, Ys, JMAX=1, I0=2, precision="f32")) / 2
+    return {
+        "value": 1.0 / (s10 * n_sigma),
+        "unit": "registrations/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "first %d of %d sigma^2 steps of one N=M=%d registration (median of 5 after "
+                  "a warm-up), FP32 OpenMP restatement of emicp_cpu; %.4f s per step, "
+                  "extrapolated to one registration" % (k, n_sigma, n, s10),
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "omp_num_threads": threads,
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "timer": "time.perf_counter (monotonic wall clock), 1 warm-up + median of 5",
+        "emicp_n1007": {
+            "registrations_per_s": 1.0 / c0_s,
+            "sigma_steps_per_s": n_sigma / c0_s,
+            "s_per_registration": c0_s,
+            "what": "full 66-step registration, uniform cube seed 0",
+            "rot_err_fp32_vs_fp64_oracle_rad": geodesic(R32.astype(float), R64),
+            "t_err_fp32_vs_fp64_oracle_over_extent": float(np.abs(t32 - t64).max()) / ext0,
+            "rot_err_gpu_vs_fp64_oracle_rad": geodesic(gpu0[:9].reshape(3, 3), R64),
+            "t_err_gpu_vs_fp64_oracle_over_extent": float(np.abs(gpu0[9:] - t64).max()) / ext0,
+        },
+        "emicp_headline_size": {
+            "sigma_steps_per_s": 1.0 / s10,
+            "registrations_per_s_extrapolated": 1.0 / (s10 * n_sigma),
+            "what": "N=M=%d, first %d sigma^2 steps; the registration figure is an "
+                    "extrapolation to %d steps" % (n, k, n_sigma),
+        },
+        "softassign_n5032": {
+            "inner_iterations_per_s": 1.0 / sa_s,
+            "registrations_per_s_extrapolated": 1.0 / (sa_s * 500),
+            "what": "not a reference path: the reference has no CPU Softassign; FP32 "
+                    "restatement of the materialised algorithm (oracle), 2 inner iterations at "
+                    "T_0 = 100, extrapolated to the 500 of a registration at the defaults",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,6 +491,9 @@ def main():
         print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(the sharding design is described with the kernels)", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; read when
+    # the first OpenMP runtime initialises, i.e. before torch is imported
+    os.environ.setdefault("OMP_PROC_BIND", "close")
 
     import torch
 
@@ -617,8 +702,8 @@ def main():
     if not args.no_cpu:
         # CPU leg: the oracle (test infrastructure) runs on the host cores --
         # the FP64 restatement of the identical iteration on pair 0 as the
-        # parity check of the GPU result, and the FP32 restatement of
-        # emicp_cpu timed as the CPU baseline
+        # parity check of the GPU result, and the FP32 restatements timed as
+        # the CPU baseline
         from oracle import oracle as O
         X, Y = data[0][0], data[0][1]
         if not args.no_check:
@@ -626,21 +711,8 @@ def main():
             out["rot_err_vs_fp64_oracle_rad"] = geodesic(Rb[0, :9].reshape(3, 3).astype(float),
                                                          Ro)
             out["t_err_vs_fp64_oracle"] = float(np.abs(Rb[0, 9:] - to).max())
-        k = 3
-        O.emicp(X, Y, precision="f32", max_iter=1)  # warm the library and pages
-        c0 = time.perf_counter()
-        O.emicp(X, Y, precision="f32", max_iter=k)
-        c_s = (time.perf_counter() - c0) / k
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = {
-            "value": 1.0 / (c_s * n_sigma),
-            "unit": "registrations/s",
-            "cores": cores,
-            "kind": "port",
-            "sample": "first %d of %d sigma^2 steps of one N=M=%d registration, FP32 OpenMP "
-                      "restatement of emicp_cpu; %.3f s per step, scaled to one registration"
-                      % (k, n_sigma, n, c_s),
-        }
+        gpu0 = drt0.cpu().numpy().astype(np.float64)
+        out["cpu_baseline"] = cpu_baseline(O, X, Y, n, n_sigma, X0, Y0, gpu0, geodesic)
     print(json.dumps(out), flush=True)
 
 
