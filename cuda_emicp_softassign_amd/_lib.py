@@ -51,8 +51,11 @@ _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _pp = C.POINTER(emsa_params)
 _ip = C.POINTER(C.c_int)
+# void (*)(int algorithm, int iteration, const float *R, const float *t, void *user)
+emsa_iteration_fn = C.CFUNCTYPE(None, C.c_int, C.c_int, _fp, _fp, C.c_void_p)
 SIGNATURES = {
     "emsa_default_params": (None, [_pp]),
+    "emsa_set_iteration_callback": (None, [emsa_iteration_fn, C.c_void_p]),
     "emsa_last_error": (C.c_char_p, []),
     "emsa_emicp_iterations": (C.c_int, [_pp]),
     "emsa_emicp": (C.c_int, [_fp, C.c_int, _fp, C.c_int, _fp, _fp, _pp]),

@@ -58,5 +58,36 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+TOOLS = os.path.join(HERE, "tools")
+BIN = os.path.join(HERE, "bin")
+# host programs: the registration harness (links libemsa) and the file tools
+PROGRAMS = {"emsa_register": True, "add_rigid_transform": False, "merge": False,
+            "pcd2ply": False}
+
+
+def build_tools(verbose: bool = False) -> str:
+    """Compiles the command-line harness and the file tools into bin/ with the
+    host compiler (no device code)."""
+    os.makedirs(BIN, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    base = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + TOOLS]
+    for prog, uses_lib in PROGRAMS.items():
+        out = os.path.join(BIN, prog)
+        cmd = base + [os.path.join(TOOLS, prog + ".cpp"), os.path.join(TOOLS, "cloud_io.cpp"),
+                      "-o", out + ".tmp"]
+        if uses_lib:
+            cmd += ["-L" + HERE, "-lemsa", "-Wl,-rpath,$ORIGIN/..",
+                    "-Wl,-rpath-link,/opt/rocm/lib", "-L/opt/rocm/lib", "-lamdhip64"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("build of %s failed" % prog)
+        os.replace(out + ".tmp", out)
+    return BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_tools(verbose="-v" in sys.argv))

@@ -11,6 +11,8 @@ namespace emsa {
 
 namespace {
 thread_local std::string g_err;
+thread_local emsa_iteration_fn g_hook = nullptr;
+thread_local void* g_hook_user = nullptr;
 std::mutex g_ctx_mu;
 // never destroyed: contexts outlive every static destructor that may still call in
 std::vector<std::unique_ptr<DeviceCtx>>& g_ctx = *new std::vector<std::unique_ptr<DeviceCtx>>();
@@ -51,6 +53,21 @@ __global__ void find_rt_kernel(const float* __restrict__ in, float* __restrict__
 
 void set_error(const std::string& msg) { g_err = msg; }
 void clear_error() { g_err.clear(); }
+
+bool iteration_hook_set() { return g_hook != nullptr; }
+
+void report_iterations(int algorithm, const double* trace, int n) {
+    const emsa_iteration_fn fn = g_hook;
+    void* const user = g_hook_user;
+    if (!fn) return;
+    for (int i = 0; i < n; ++i) {
+        const double* o = trace + (size_t)i * EMSA_TRACE_W;
+        float R[9], t[3];
+        for (int k = 0; k < 9; ++k) R[k] = (float)o[17 + k];
+        for (int k = 0; k < 3; ++k) t[k] = (float)o[26 + k];
+        fn(algorithm, i, R, t, user);
+    }
+}
 
 void DevBuf::release() {
     if (ptr) (void)hipFree(ptr);
@@ -408,13 +425,25 @@ int emsa_emicp_iterations(const emsa_params* p) {
     return (int)sigma_schedule(*p).size();
 }
 
+void emsa_set_iteration_callback(emsa_iteration_fn fn, void* user) {
+    g_hook = fn;
+    g_hook_user = fn ? user : nullptr;
+}
+
 int emsa_emicp(const float* X, int N, const float* Y, int M, float* R, float* t,
                const emsa_params* p) {
     clear_error();
     if (!p) return set_error("emicp: null params"), EMSA_EINVAL;
     const int rc = check_schedule(*p, "emicp");
     if (rc != EMSA_OK) return rc;
-    return emicp_host(X, N, Y, M, R, t, *p, sigma_schedule(*p), nullptr, 0, nullptr, nullptr);
+    const std::vector<float> sig = sigma_schedule(*p);
+    if (!iteration_hook_set())
+        return emicp_host(X, N, Y, M, R, t, *p, sig, nullptr, 0, nullptr, nullptr);
+    std::vector<double> trace(std::max<size_t>(sig.size(), 1) * EMSA_TRACE_W);
+    const int hrc = emicp_host(X, N, Y, M, R, t, *p, sig, trace.data(), (int)sig.size(), nullptr,
+                               nullptr);
+    if (hrc == EMSA_OK) report_iterations(EMSA_ALG_EMICP, trace.data(), (int)sig.size());
+    return hrc;
 }
 
 int emsa_emicp_traced(const float* X, int N, const float* Y, int M, float* R, float* t,
@@ -447,7 +476,12 @@ int emsa_softassign(const float* X, int N, const float* Y, int M, float* R, floa
                     const emsa_params* p, int variant) {
     clear_error();
     if (!p) return set_error("softassign: null params"), EMSA_EINVAL;
-    return softassign_host(X, N, Y, M, R, t, *p, variant, nullptr, 0);
+    if (!iteration_hook_set()) return softassign_host(X, N, Y, M, R, t, *p, variant, nullptr, 0);
+    const int nrec = std::max(0, p->JMAX) * std::max(0, p->I0);
+    std::vector<double> trace(std::max(nrec, 1) * (size_t)EMSA_TRACE_W);
+    const int rc = softassign_host(X, N, Y, M, R, t, *p, variant, trace.data(), nrec);
+    if (rc == EMSA_OK) report_iterations(EMSA_ALG_SOFTASSIGN, trace.data(), nrec);
+    return rc;
 }
 
 int emsa_softassign_traced(const float* X, int N, const float* Y, int M, float* R, float* t,

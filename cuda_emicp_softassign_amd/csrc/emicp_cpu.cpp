@@ -197,7 +197,15 @@ int emsa_emicp_cpu(const float* X, int N, const float* Y, int M, float* R, float
         if (!std::isfinite(R[i])) return set_error("emicp_cpu: non-finite R"), EMSA_ENONFINITE;
     for (int i = 0; i < 3; ++i)
         if (!std::isfinite(t[i])) return set_error("emicp_cpu: non-finite t"), EMSA_ENONFINITE;
+    if (!trace && iteration_hook_set()) {
+        std::vector<double> own((size_t)std::max<size_t>(sigma_schedule(*p).size(), 1) *
+                                EMSA_TRACE_W);
+        rc = emicp_cpu_run(X, N, Y, M, R, t, *p, own.data(), (int)(own.size() / EMSA_TRACE_W));
+        if (rc >= 0) report_iterations(EMSA_ALG_EMICP_CPU, own.data(), rc);
+        return rc;
+    }
     rc = emicp_cpu_run(X, N, Y, M, R, t, *p, trace, trace ? trace_cap : 0);
+    if (rc >= 0 && trace) report_iterations(EMSA_ALG_EMICP_CPU, trace, std::min(rc, trace_cap));
     return rc < 0 ? rc : std::min(rc, trace ? trace_cap : rc);
 }
 

@@ -15,6 +15,10 @@ namespace emsa {
 
 void set_error(const std::string& msg);
 void clear_error();
+// The calling thread's per-iteration hook (emsa_set_iteration_callback), and
+// its replay of n trace records (R at record[17..25], t at [26..28]).
+bool iteration_hook_set();
+void report_iterations(int algorithm, const double* trace, int n);
 bool finite_params(const emsa_params& p);
 bool finite_cloud(const float* P, int n);
 

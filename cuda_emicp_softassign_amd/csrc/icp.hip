@@ -402,7 +402,14 @@ int emsa_icp(const float* X, int N, const float* Y, int M, float* R, float* t,
              const emsa_params* p) {
     clear_error();
     if (!p) return set_error("icp: null params"), EMSA_EINVAL;
-    return icp_host(X, N, Y, M, R, t, p->maxIteration, nullptr, 0, nullptr, nullptr, "icp");
+    if (!iteration_hook_set())
+        return icp_host(X, N, Y, M, R, t, p->maxIteration, nullptr, 0, nullptr, nullptr, "icp");
+    const int nrec = std::max(0, p->maxIteration);
+    std::vector<double> trace(std::max(nrec, 1) * (size_t)EMSA_TRACE_W);
+    const int rc =
+        icp_host(X, N, Y, M, R, t, p->maxIteration, trace.data(), nrec, nullptr, nullptr, "icp");
+    if (rc == EMSA_OK) report_iterations(EMSA_ALG_ICP, trace.data(), nrec);
+    return rc;
 }
 
 int emsa_icp_traced(const float* X, int N, const float* Y, int M, float* R, float* t,

@@ -48,6 +48,45 @@ void report(const char* fn, int rc) {
     if (rc != EMSA_OK) std::fprintf(stderr, "%s failed (%d): %s\n", fn, rc, emsa_last_error());
 }
 
+#ifdef EMSA_HAVE_PCL_VIEWER
+// The reference redraws the transformed source after every iteration when
+// the viewer is on (src/emicp.cu:652-664, src/softassign.cu:805-815,
+// src/icp.cpp:272-282); here the per-iteration hook of the C ABI does it.
+void redraw(int, int, const float* R, const float* t, void* user) {
+    const registrationParameters& param = *static_cast<const registrationParameters*>(user);
+    Eigen::Matrix4f T = Eigen::Matrix4f::Identity();
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) T(r, c) = R[3 * r + c];
+        T(r, 3) = t[r];
+    }
+    pcl::transformPointCloud(*param.cloud_source, *param.cloud_source_trans, T);
+    param.viewer->updatePointCloud(param.cloud_source_trans, *param.source_trans_color,
+                                   "source trans");
+    param.viewer->spinOnce();
+}
+
+// Installs the redraw for one call when the caller's parameters carry a
+// viewer and -noviewer is off (it replaces a hook the caller set on this
+// thread for that call, and removes it afterwards).
+struct ViewerScope {
+    bool on;
+    explicit ViewerScope(const registrationParameters& p)
+        : on(!p.noviewer && p.viewer && p.cloud_source && p.cloud_source_trans &&
+             p.source_trans_color) {
+        if (on) emsa_set_iteration_callback(redraw, const_cast<registrationParameters*>(&p));
+    }
+    ~ViewerScope() {
+        if (on) emsa_set_iteration_callback(nullptr, nullptr);
+    }
+};
+#else
+// Without PCL's viewer there is nothing to redraw; a hook the caller set with
+// emsa_set_iteration_callback stays in effect.
+struct ViewerScope {
+    explicit ViewerScope(const registrationParameters&) {}
+};
+#endif
+
 }  // namespace
 
 void emsa_set_softassign_variant(int variant) { g_variant = variant; }
@@ -57,6 +96,7 @@ void emicp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
            const registrationParameters& param) {
     const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
     const emsa_params p = to_params(param);
+    const ViewerScope viewer(param);
     report("emicp", emsa_emicp(X.data(), (int)(X.size() / 3), Y.data(), (int)(Y.size() / 3), h_R,
                                h_t, &p));
 }
@@ -66,6 +106,7 @@ void emicp_cpu(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
                const registrationParameters& param) {
     const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
     const emsa_params p = to_params(param);
+    const ViewerScope viewer(param);
     const int rc = emsa_emicp_cpu(X.data(), (int)(X.size() / 3), Y.data(), (int)(Y.size() / 3),
                                   h_R, h_t, &p, nullptr, 0);
     report("emicp_cpu", rc < 0 ? rc : EMSA_OK);
@@ -76,6 +117,7 @@ void icp(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
          const registrationParameters& param) {
     const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
     const emsa_params p = to_params(param);
+    const ViewerScope viewer(param);
     report("icp", emsa_icp(X.data(), (int)(X.size() / 3), Y.data(), (int)(Y.size() / 3), h_R, h_t,
                            &p));
 }
@@ -85,6 +127,7 @@ void softassign(const pcl::PointCloud<pcl::PointXYZ>::Ptr cloud_target,
                 const registrationParameters& param) {
     const std::vector<float> X = to_soa(cloud_target), Y = to_soa(cloud_source);
     const emsa_params p = to_params(param);
+    const ViewerScope viewer(param);
     report("softassign", emsa_softassign(X.data(), (int)(X.size() / 3), Y.data(),
                                          (int)(Y.size() / 3), h_R, h_t, &p, g_variant.load()));
 }

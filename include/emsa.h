@@ -69,6 +69,21 @@ const char *emsa_last_error(void);
  * in float exactly as the reference loop (src/emicp.cu:443, :668). */
 int emsa_emicp_iterations(const emsa_params *p);
 
+/* Per-iteration hook: the counterpart of the reference's viewer update inside
+ * its loops (src/emicp.cu:652-664, src/softassign.cu:805-815,
+ * src/icp.cpp:272-282), which transforms the source cloud with the current
+ * (R, t) and redraws it. Once set on a thread, emsa_emicp, emsa_softassign,
+ * emsa_icp and emsa_emicp_cpu called from that thread call fn once per
+ * iteration (sigma^2 step, inner iteration, ICP iteration), in order, with
+ * that iteration's R (9, row-major) and t (3). The GPU loops have no host
+ * round trip per iteration, so the calls come after the registration has
+ * finished and before the entry point returns; none come when it fails.
+ * fn = NULL removes the hook. algorithm: EMSA_ALG_*. */
+enum { EMSA_ALG_EMICP = 1, EMSA_ALG_SOFTASSIGN = 2, EMSA_ALG_ICP = 3, EMSA_ALG_EMICP_CPU = 4 };
+typedef void (*emsa_iteration_fn)(int algorithm, int iteration, const float *R, const float *t,
+                                  void *user);
+void emsa_set_iteration_callback(emsa_iteration_fn fn, void *user);
+
 /* ---- Host-buffer entry points (the drop-in boundary) -------------------- */
 
 /* EM-ICP on the GPU. Replaces emicp() (src/emicp.cu:247-716; declared at

@@ -1,7 +1,8 @@
 // A caller written against the reference API (src/3dregistration.h): builds
 // two clouds, runs emicp(), emicp_cpu(), softassign() and icp() through the
-// drop-in header and checks that the known rigid transform comes back. Exit
-// status 0 on success.
+// drop-in header and checks that the known rigid transform comes back, and
+// that the per-iteration hook sees every ICP iteration. Exit status 0 on
+// success.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -100,10 +101,32 @@ int main() {
     }
     emsa_set_softassign_variant(EMSA_SOFTASSIGN_MATERIALISED);
 
-    // icp(): noiseless pair, every source point has its exact partner
+    // icp(): noiseless pair, every source point has its exact partner; a
+    // viewer-style hook sees every iteration's transform, the last one equal
+    // to the result
+    struct Seen {
+        int n = 0, alg = 0;
+        float R[9], t[3];
+    } seen;
+    emsa_set_iteration_callback(
+        [](int alg, int, const float* R, const float* t, void* u) {
+            Seen* s = static_cast<Seen*>(u);
+            ++s->n;
+            s->alg = alg;
+            for (int i = 0; i < 9; ++i) s->R[i] = R[i];
+            for (int i = 0; i < 3; ++i) s->t[i] = t[i];
+        },
+        &seen);
     float i_R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, i_t[3] = {0, 0, 0};
     param.maxIteration = 60;
     icp(target, source, i_R, i_t, param);
+    emsa_set_iteration_callback(nullptr, nullptr);
+    std::printf("iteration hook: %d calls\n", seen.n);
+    if (seen.n != 60 || seen.alg != EMSA_ALG_ICP) return 7;
+    for (int i = 0; i < 9; ++i)
+        if (seen.R[i] != i_R[i]) return 7;
+    for (int i = 0; i < 3; ++i)
+        if (seen.t[i] != i_t[i]) return 7;
     printRT(i_R, i_t);
     double err3 = 0;
     for (int i = 0; i < 9; ++i) err3 = std::fmax(err3, std::fabs(i_R[i] - Rt[i]));
