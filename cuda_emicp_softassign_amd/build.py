@@ -28,14 +28,19 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, host_flags=(), out: str = LIB) -> str:
+    """Compiles every source and links `out` (default: the in-tree
+    libemsa.so). host_flags are added to the host side of every compile and
+    to the link (e.g. a sanitizer build: each flag preceded by -Xarch_host)."""
     objs = []
-    os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    objdir = os.path.join(HERE, "_obj") if out == LIB else out + "_obj"
+    os.makedirs(objdir, exist_ok=True)
     for src in sources():
-        obj = os.path.join(HERE, "_obj", os.path.basename(src) + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-c", src, "-o", obj]
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + list(host_flags) + ["-c", src, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-c", src, "-o", obj]
+            cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + list(host_flags) + \
+                ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -47,15 +52,16 @@ def build(verbose: bool = False) -> str:
         if warn and verbose:
             sys.stderr.write(warn + "\n")
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + \
-        ["-Wl,--no-undefined", "-Wl,-soname,libemsa.so"]
+        list(host_flags) + ["-Wl,-soname,libemsa.so"] + \
+        ([] if host_flags else ["-Wl,--no-undefined"])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libemsa.so failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 TOOLS = os.path.join(HERE, "tools")

@@ -19,7 +19,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "_build", "libemsa_oracle.so")
+# EMSA_ORACLE_LIB: another build of the same sources (the sanitizer test's)
+_SO = os.environ.get("EMSA_ORACLE_LIB") or os.path.join(_HERE, "_build", "libemsa_oracle.so")
 TRACE_W = 32
 _lib = None
 
