@@ -20,6 +20,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for all device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# per source: the Softassign log-domain passes chain short matrix-core
+# products into VALU work; their results go straight to VGPRs
+SOURCE_FLAGS = {"softassign.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall",
           "-Wno-unused-function", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
@@ -37,9 +40,10 @@ def build(verbose: bool = False, host_flags=(), out: str = LIB) -> str:
     os.makedirs(objdir, exist_ok=True)
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + list(host_flags) + ["-c", src, "-o", obj]
+        extra = SOURCE_FLAGS.get(os.path.basename(src), [])
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + extra + list(host_flags) + ["-c", src, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + list(host_flags) + \
+            cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + extra + list(host_flags) + \
                 ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -160,7 +160,7 @@ struct DeviceCtx {
     EmWork em;
     // softassign workspaces (sastep: staging of emsa_softassign_step's slack
     // vectors and taps)
-    DevBuf K, u, v, orow, ocol, colpart, samom, sastate, sastep;
+    DevBuf K, u, v, orow, ocol, colpart, samom, sastate, sastep, sapack;
     // ICP workspaces: per-tile nearest candidates, row-block moments, state,
     // per-row neighbour and distance (the correspondence query)
     DevBuf icpDist, icpIdx, icpMom, icpState, icpRowIdx, icpRowDist;

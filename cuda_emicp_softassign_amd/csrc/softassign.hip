@@ -54,6 +54,20 @@ constexpr int kSaColsPB = 256;               // columns per column-pass workgrou
 constexpr int kSaMomW = 22;                  // per-workgroup moments of the final pass
 constexpr int kSaMomPad = 24;                // moment rows allocated (a multiple of kSaWaves)
 constexpr float kSaSentinel = 1.0e15f;
+constexpr float kSaMasked = -1.0e30f;        // exponent term of a padded row or column
+constexpr int kSaMfItems = 4;                // rows (columns) per workgroup of a matrix-core pass
+constexpr int kSaMfWaves = 4;                // its waves: they split the inner tiles
+constexpr int kSaMfTile = 64;                // columns (rows) per matrix-core tile
+constexpr int kSaMfU = 4;                    // tiles per wave and step (loads of the next step in flight)
+constexpr int kSaMfStride = kSaMfWaves * kSaMfU;  // tiles per step of a workgroup
+// the packs are padded (masked entries) to whole steps plus one step, so
+// every load of a pass, the prefetch past the last step included, is in bounds
+__host__ __device__ constexpr int sa_mf_tiles(int n) {
+    return ((n + kSaMfTile - 1) / kSaMfTile + kSaMfStride - 1) / kSaMfStride * kSaMfStride;
+}
+__host__ __device__ constexpr int sa_mf_entries(int n) {
+    return (sa_mf_tiles(n) + kSaMfStride) * kSaMfTile;
+}
 // The log domain evaluates the exponent expanded (a_r + b_c + x'_r . p_c)
 // at the temperatures where k1 max_r |x_r|^2 is at most this: the expanded
 // terms then round to about 1e-5 of the exponent at worst. Where the
@@ -113,16 +127,59 @@ struct SaBufs {
     double* tapA;
     double* tapB;
     double* tapS;
+    // log domain, expanded exponent on the matrix cores: per-column terms
+    // {beta_c = k0 + g_c - k1 |p_c|^2, p_c} [ldk] for the row passes and
+    // per-row terms {alpha'_r = k0 + f_r - k1 |x_r|^2, 2 k1 x_r} for the
+    // column passes; both padded to whole steps of a pass (sa_mf_entries) with
+    // masked entries (beta, alpha' = kSaMasked, the rest 0)
+    float4* cpack;
+    float4* rpack;
 };
 
+// The transformed sources p_c = R y_c + t (FP64, rounded) of the coming
+// inner iteration with fresh column potentials (g = 0): P, and the column
+// terms of the matrix-core row passes for exponent constants (k1, k0).
+__device__ void sa_prep_columns(const SaBufs& b, int M, int ldk, float k1, float k0,
+                                const double* Rm, const double* tm) {
+    for (int c = threadIdx.x; c < ldk; c += blockDim.x) {
+        float p[3];
+        if (c < M) {
+            const double y0 = b.Y[c], y1 = b.Y[(size_t)M + c], y2 = b.Y[2 * (size_t)M + c];
+            for (int a = 0; a < 3; ++a)
+                p[a] = (float)(Rm[3 * a] * y0 + Rm[3 * a + 1] * y1 + Rm[3 * a + 2] * y2 + tm[a]);
+        } else {
+            p[0] = p[1] = p[2] = kSaSentinel;
+        }
+        for (int a = 0; a < 3; ++a) b.P[(size_t)a * ldk + c] = p[a];
+        if (b.cpack)
+            b.cpack[c] = c < M ? make_float4(fmaf(-k1, fmaf(p[0], p[0], fmaf(p[1], p[1], p[2] * p[2])),
+                                                  k0 + 0.f),
+                                             p[0], p[1], p[2])
+                               : make_float4(kSaMasked, 0.f, 0.f, 0.f);
+    }
+}
+
 __global__ void __launch_bounds__(1024)
-sa_setup_kernel(SaBufs b, int N, int M, int ldk, const float* __restrict__ Rt, float moutlier) {
+sa_setup_kernel(SaBufs b, int N, int M, int ldk, const float* __restrict__ Rt, float moutlier,
+                int prep, float k1, float k0) {
     if (threadIdx.x < 12) b.state[threadIdx.x] = (double)Rt[threadIdx.x];
     if (threadIdx.x == 12) b.state[kStStatus] = 0.0;
     for (int r = threadIdx.x; r < N; r += blockDim.x) b.orow[r] = moutlier;
     for (int c = threadIdx.x; c < M; c += blockDim.x) b.ocol[c] = moutlier;
     for (int c = threadIdx.x; c < ldk; c += blockDim.x)
         for (int a = 0; a < 3; ++a) b.Yp[(size_t)a * ldk + c] = (c < M) ? b.Y[(size_t)a * M + c] : 0.f;
+    if (b.rpack) {
+        for (int r = N + threadIdx.x; r < sa_mf_entries(N); r += blockDim.x)
+            b.rpack[r] = make_float4(kSaMasked, 0.f, 0.f, 0.f);
+        for (int c = ldk + threadIdx.x; c < sa_mf_entries(M); c += blockDim.x)
+            b.cpack[c] = make_float4(kSaMasked, 0.f, 0.f, 0.f);
+    }
+    if (prep) {
+        double Rm[9], tm[3];
+        for (int i = 0; i < 9; ++i) Rm[i] = (double)Rt[i];
+        for (int i = 0; i < 3; ++i) tm[i] = (double)Rt[9 + i];
+        sa_prep_columns(b, M, ldk, k1, k0, Rm, tm);
+    }
 }
 
 __global__ void sa_output_kernel(const double* __restrict__ state, float* __restrict__ Rt) {
@@ -649,7 +706,8 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
 constexpr int kSaSolveWaves = 16;
 
 __global__ void __launch_bounds__(kWave * kSaSolveWaves)
-sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
+sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter, int M, int ldk, int prep,
+                float k1n, float k0n) {
     constexpr int kParts = kSaSolveWaves / kSaWaves;
     static_assert(kParts * kSaWaves == kSaSolveWaves, "whole quantity classes per range");
     const int tid = threadIdx.x;
@@ -683,8 +741,15 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
     for (int k = 0; k < kPer; ++k) acc[k][0] = (acc[k][0] + acc[k][1]) + (acc[k][2] + acc[k][3]);
     const bool byWave = (part == 0 && qc < 3);  // component qc of sum b y
     double by = 0.0;
-    if (byWave)
-        for (int c = lane; c < nfin; c += kWave) by += b.cby[(size_t)qc * nfin + c];
+    if (byWave) {  // four partial sums per lane so the loads are in flight together
+        double bq[4] = {0.0, 0.0, 0.0, 0.0};
+        int c = lane;
+        for (; c + 3 * kWave < nfin; c += 4 * kWave)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) bq[u] += b.cby[(size_t)qc * nfin + c + u * kWave];
+        for (; c < nfin; c += kWave) bq[0] += b.cby[(size_t)qc * nfin + c];
+        by = (bq[0] + bq[1]) + (bq[2] + bq[3]);
+    }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) acc[k][0] = wave_sum(acc[k][0]);
     by = wave_sum(by);
@@ -743,6 +808,244 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter) {
             o[31] = ok ? 0.0 : 1.0;
         }
     }
+    // the next inner iteration's transformed sources and column terms, for
+    // the matrix-core passes (exponent constants of its temperature)
+    if (prep) {
+        __syncthreads();
+        double Rm[9], tm[3];
+        for (int i = 0; i < 9; ++i) Rm[i] = b.state[kStR + i];
+        for (int i = 0; i < 3; ++i) tm[i] = b.state[kStT + i];
+        sa_prep_columns(b, M, ldk, k1n, k0n, Rm, tm);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Log domain with the expanded exponent on the matrix cores. The exponent of
+// element (r, c) is a rank-5 product,
+//   E_rc = alpha_r * 1 + 1 * beta_c + (2 k1 x_r) . p_c,
+// evaluated with five v_mfma_f32_4x4x1f32 (16 blocks of 4 x 4, contraction
+// depth 1 each, accumulator chained), in the order the VALU form adds the
+// terms. A wave owns kSaMfItems = 4 outer items (rows in a row pass, columns
+// in a column pass) and streams every inner item (columns / rows) in tiles of
+// 64, one inner item per lane: lane l supplies the inner item's terms (A
+// operand), lane 4b + j the outer item j's (B operand, the same for every b),
+// and lane 4b + j receives E for inner items 4b..4b+3 of the tile and outer
+// item j. Each lane adds exp2(E) of its 4 elements (one FP64 sum per tile),
+// and the 16 lanes of an outer item are added at the end, so a wave holds
+// complete row (column) sums: the normalisation, the slack update and the
+// terms the next pass needs are written by the wave itself, with no partial
+// sums and no finalize kernel.
+// ---------------------------------------------------------------------------
+typedef float sa_f4 __attribute__((ext_vector_type(4)));
+
+// E for one tile: inner terms (ia: alpha or 1, ib: 1 or beta, ip: 3), outer (oa, ob, op: 3)
+__device__ inline sa_f4 sa_exponent(float ia, float ib, const float (&ip)[3], float oa, float ob,
+                                    const float (&op)[3]) {
+    sa_f4 d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(ia, oa, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(ib, ob, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(ip[0], op[0], d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(ip[1], op[1], d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_4x4x1f32(ip[2], op[2], d, 0, 0, 0);
+    return d;
+}
+
+// Sum over the 16 lanes holding the same outer item (lane bits 2..5).
+__device__ inline double sa_item_sum(double v) {
+#pragma unroll
+    for (int m = 4; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Row passes (MODE 0: first of an inner iteration, 1: Sinkhorn). Columns in
+// tiles of 64 from cpack. The final pass (its four weighted sums per element
+// and the moments) is the VALU row pass, which reads the same P, v and
+// weights.
+template <int MODE>
+__global__ void __launch_bounds__(kWave * kSaMfWaves)
+sa_mrows_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0) {
+    static_assert(MODE == 0 || MODE == 1, "Sinkhorn row passes; the final pass is the VALU one");
+    constexpr int U = kSaMfU;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int j = lane & (kSaMfItems - 1);
+    const int r0 = blockIdx.x * kSaMfItems;
+    const int r = r0 + j;
+    // outer terms of row r: alpha_r = -k1 |x_r|^2, 1, 2 k1 x_r (masked past N)
+    float x[3] = {0.f, 0.f, 0.f};
+    if (r < N) {
+        x[0] = b.X[r];
+        x[1] = b.X[(size_t)N + r];
+        x[2] = b.X[2 * (size_t)N + r];
+    }
+    const float oa = (r < N) ? -k1 * fmaf(x[0], x[0], fmaf(x[1], x[1], x[2] * x[2])) : kSaMasked;
+    const float op[3] = {2.f * k1 * x[0], 2.f * k1 * x[1], 2.f * k1 * x[2]};
+    // the epilogue's operands, loaded now so that they arrive during the loop
+    const bool fin = threadIdx.x < kSaMfItems && r < N;
+    const float su = (fin && MODE != 0) ? b.u[r] : 0.f;
+    const float orow = fin ? b.orow[r] : 0.f;
+    double acc = 0.0;
+    // wave w takes tiles w, w + W, ... in steps of U tiles; the next step's
+    // loads (into the other buffer) are issued before the current step's
+    // arithmetic. The packs are padded to whole steps, so no load is guarded.
+    const int tiles = (r0 < N) ? sa_mf_tiles(M) : 0;  // block-uniform
+    auto load = [&](int t0, float4 (&cp)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) cp[u] = b.cpack[(size_t)(t0 + u * kSaMfWaves) * kSaMfTile + lane];
+    };
+    auto step = [&](const float4 (&cp)[U]) {
+        sa_f4 d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float ip[3] = {cp[u].y, cp[u].z, cp[u].w};
+            d[u] = sa_exponent(1.f, cp[u].x, ip, oa, 1.f, op);
+        }
+        float sq = 0.f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float e[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) e[i] = __builtin_amdgcn_exp2f(d[u][i]);
+            sq += (e[0] + e[1]) + (e[2] + e[3]);
+        }
+        acc += (double)sq;
+    };
+    float4 ca[U], cb[U];
+    if (tiles > 0) load(wave, ca);
+    for (int t0 = wave; t0 < tiles; t0 += 2 * kSaMfStride) {
+        load(t0 + kSaMfStride, cb);
+        step(ca);
+        if (t0 + kSaMfStride >= tiles) break;  // uniform
+        load(t0 + 2 * kSaMfStride, ca);
+        step(cb);
+    }
+    // the 16 lanes of each row, then the waves in a fixed order
+    __shared__ double red[kSaMfWaves][kSaMfItems];
+    acc = sa_item_sum(acc);
+    if (lane < kSaMfItems) red[wave][j] = acc;
+    __syncthreads();
+    if (threadIdx.x >= kSaMfItems) return;  // thread j < 4 (wave 0) finishes row r
+    double tot = red[0][j];
+#pragma unroll
+    for (int w2 = 1; w2 < kSaMfWaves; ++w2) tot += red[w2][j];
+    // a_r = (row scale) * sum + o_row[r]; scale /= a_r; o_row /= a_r; and the
+    // row terms of the column passes
+    if (r < N) {
+        const float scale = exp2f(su);
+        const float a = scale * (float)tot + orow;
+        b.orow[r] = orow / a;
+        const float un = su - log2f(a);
+        b.u[r] = un;
+        if (b.tapA) b.tapA[r] = (double)a;
+        b.rpack[r] = make_float4(fmaf(-k1, fmaf(x[0], x[0], fmaf(x[1], x[1], x[2] * x[2])), k0 + un),
+                                 op[0], op[1], op[2]);
+    }
+}
+
+// Column passes (FIRST: the first sweep of an inner iteration, fresh column
+// scale). Rows in tiles of 64 from rpack. The workgroup's 4 columns get
+// b_c = (column scale) * sum + o_col[c], the new potential, the column terms
+// of the next row pass, and after the last sweep the final pass's weights;
+// the workgroup's sum_c b_c y_c (columns in order) goes to cby.
+template <bool FIRST>
+__global__ void __launch_bounds__(kWave * kSaMfWaves)
+sa_mcols_kernel(SaBufs b, int N, int M, int ldk, float k1, float k0, int last) {
+    constexpr int U = kSaMfU;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int j = lane & (kSaMfItems - 1);
+    const int quad = lane >> 2;
+    const int c0 = blockIdx.x * kSaMfItems;
+    const int c = c0 + j;
+    // outer terms of column c: 1, beta'_c = -k1 |p_c|^2, p_c (masked past M)
+    const float p[3] = {b.P[c], b.P[(size_t)ldk + c], b.P[2 * (size_t)ldk + c]};
+    const float ob =
+        (c < M) ? -k1 * fmaf(p[0], p[0], fmaf(p[1], p[1], p[2] * p[2])) : kSaMasked;
+    const float op[3] = {c < M ? p[0] : 0.f, c < M ? p[1] : 0.f, c < M ? p[2] : 0.f};
+    // the epilogue's operands, loaded now so that they arrive during the loop
+    const bool fin = threadIdx.x < kSaMfItems && c < M;
+    const float sv = (fin && !FIRST) ? b.v[c] : 0.f;
+    const float ocol = fin ? b.ocol[c] : 0.f;
+    const float y[3] = {fin ? b.Y[c] : 0.f, fin ? b.Y[(size_t)M + c] : 0.f,
+                        fin ? b.Y[2 * (size_t)M + c] : 0.f};
+    double acc = 0.0;
+    const int tiles = (c0 < M) ? sa_mf_tiles(N) : 0;  // block-uniform
+    auto load = [&](int t0, float4 (&rp)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) rp[u] = b.rpack[(size_t)(t0 + u * kSaMfWaves) * kSaMfTile + lane];
+    };
+    auto step = [&](const float4 (&rp)[U]) {
+        sa_f4 d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float ip[3] = {rp[u].y, rp[u].z, rp[u].w};
+            // beta'_c + alpha'_r (equal to alpha'_r + beta'_c; the first
+            // product is the same for every tile)
+            d[u] = sa_exponent(1.f, rp[u].x, ip, ob, 1.f, op);
+        }
+        float sq = 0.f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float e[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) e[i] = __builtin_amdgcn_exp2f(d[u][i]);
+            sq += (e[0] + e[1]) + (e[2] + e[3]);
+        }
+        acc += (double)sq;
+    };
+    float4 ra[U], rb[U];
+    if (tiles > 0) load(wave, ra);
+    for (int t0 = wave; t0 < tiles; t0 += 2 * kSaMfStride) {
+        load(t0 + kSaMfStride, rb);
+        step(ra);
+        if (t0 + kSaMfStride >= tiles) break;  // uniform
+        load(t0 + 2 * kSaMfStride, ra);
+        step(rb);
+    }
+    __shared__ double red[kSaMfWaves][kSaMfItems];
+    acc = sa_item_sum(acc);
+    if (quad == 0) red[wave][j] = acc;
+    __syncthreads();
+    if (threadIdx.x >= kSaMfItems) return;  // thread j < 4 finishes column c
+    double tot = red[0][j];
+#pragma unroll
+    for (int w2 = 1; w2 < kSaMfWaves; ++w2) tot += red[w2][j];
+    double by[3] = {0.0, 0.0, 0.0};
+    if (c < M) {
+        const float scale = exp2f(sv);
+        const float bc = (float)(scale * tot) + ocol;
+        if (b.tapB) b.tapB[c] = (double)bc;
+        b.ocol[c] = ocol / bc;
+        const float vn = sv - log2f(bc);
+        b.v[c] = vn;
+        b.bcol[c] = bc;
+        const float y0 = y[0], y1 = y[1], y2 = y[2];
+        by[0] = (double)bc * y0;
+        by[1] = (double)bc * y1;
+        by[2] = (double)bc * y2;
+        if (last) {
+            b.wfin[c] = 1.f;
+            b.wfin[(size_t)ldk + c] = bc * y0;
+            b.wfin[2 * (size_t)ldk + c] = bc * y1;
+            b.wfin[3 * (size_t)ldk + c] = bc * y2;
+        }
+        b.cpack[c] = make_float4(fmaf(-k1, fmaf(p[0], p[0], fmaf(p[1], p[1], p[2] * p[2])),
+                                      k0 + vn),
+                                 p[0], p[1], p[2]);
+    } else {
+        b.v[c] = 0.f;
+        b.bcol[c] = 0.f;
+        if (last)
+            for (int q = 0; q < 4; ++q) b.wfin[(size_t)q * ldk + c] = 0.f;
+        b.cpack[c] = make_float4(kSaMasked, 0.f, 0.f, 0.f);
+    }
+    // columns j = 0..3 are lanes 0..3 of wave 0: add in order
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double b1 = __shfl(by[i], 1, kWave), b2 = __shfl(by[i], 2, kWave),
+                     b3 = __shfl(by[i], 3, kWave);
+        if (j == 0) b.cby[(size_t)i * gridDim.x + blockIdx.x] = ((by[i] + b1) + b2) + b3;
+    }
 }
 
 template <int S>
@@ -794,6 +1097,11 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     const int finalBlocks = LOGV ? pl.rowBlocks : (N + kSaFinalRows - 1) / kSaFinalRows;
     const int nmom = finalBlocks;
     const int finBlocks = pl.ldk / kSaFinCols;  // column-finalize workgroups
+    // matrix-core passes (log domain, expanded exponent): workgroups of the
+    // row and column passes; the final row pass writes mfRows moment rows and
+    // each column pass mfCols partial sums of sum_c b_c y_c
+    const int mfRows = (N + kSaMfItems - 1) / kSaMfItems;
+    const int mfCols = pl.ldk / kSaMfItems;
     int rc;
     SaBufs b{};
     b.X = dX;
@@ -825,15 +1133,38 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
     if ((rc = ctx.colpart.reserve(sizeof(float) * partRows * pl.ldk)) != EMSA_OK) return rc;
     b.cpart = ctx.colpart.as<float>();
     if ((rc = ctx.samom.reserve(sizeof(double) *
-                                ((size_t)nmom * kSaMomPad + (size_t)finBlocks * 3))) != EMSA_OK)
+                                ((size_t)nmom * kSaMomPad +
+                                 (size_t)std::max(finBlocks, mfCols) * 3))) != EMSA_OK)
         return rc;
     b.rmom = ctx.samom.as<double>();
     b.cby = b.rmom + (size_t)nmom * kSaMomPad;
     if ((rc = ctx.sastate.reserve(sizeof(double) * kStateW)) != EMSA_OK) return rc;
     b.state = ctx.sastate.as<double>();
     b.trace = trace;
-    hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
-                       p.moutlier);
+    if (LOGV) {
+        const size_t ce = (size_t)sa_mf_entries(M), re = (size_t)sa_mf_entries(N);
+        if ((rc = ctx.sapack.reserve(sizeof(float4) * (std::max(ce, (size_t)pl.ldk) + re))) !=
+            EMSA_OK)
+            return rc;
+        b.cpack = ctx.sapack.as<float4>();
+        b.rpack = b.cpack + std::max(ce, (size_t)pl.ldk);
+    }
+    float T = step ? step->T : p.T_0;
+    // exponent constants and form at temperature T (log domain: expanded
+    // while k1 max|x|^2 <= kSaExpandMax, on the matrix cores)
+    auto consts = [&](float Tt, float& k1, float& k0, bool& expand) {
+        const double k1d = kLog2e / (double)Tt;
+        k1 = (float)k1d;
+        k0 = (float)((double)p.alpha * k1d - 0.5 * std::log2((double)Tt));
+        expand = LOGV && k1 * xmax2 <= kSaExpandMax;
+    };
+    {
+        float k1, k0;
+        bool expand;
+        consts(T, k1, k0, expand);
+        hipLaunchKernelGGL(sa_setup_kernel, dim3(1), dim3(1024), 0, s, b, N, M, pl.ldk, dRtIn,
+                           p.moutlier, expand ? 1 : 0, k1, k0);
+    }
     EMSA_HIP_TRY(hipGetLastError());
     if (step) {  // the caller's slack state instead of moutlier
         EMSA_HIP_TRY(hipMemcpyAsync(b.orow, step->orow, sizeof(float) * N,
@@ -846,13 +1177,12 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
         fgrid((unsigned)finBlocks);
     const dim3 blk(kWave * kSaWaves), wblk(kWave * kSaFinWavesWide);
     const dim3 xgrid((unsigned)pl.fuseBlocks), xblk(kWave * kSaFuseWaves);
-    float T = step ? step->T : p.T_0;
+    const dim3 mrgrid((unsigned)mfRows), mcgrid((unsigned)mfCols), mblk(kWave * kSaMfWaves);
     int it = 0;
     for (int Titer = 1; Titer <= JMAX; ++Titer) {
-        const double k1d = kLog2e / (double)T;
-        const float k1 = (float)k1d;
-        const float k0 = (float)((double)p.alpha * k1d - 0.5 * std::log2((double)T));
-        const bool expand = k1 * xmax2 <= kSaExpandMax;  // log domain: the exponent's form
+        float k1, k0;
+        bool expand;
+        consts(T, k1, k0, expand);
         auto log_rows = [&](int mode) {
             if (expand) launch_log_rows<true>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
             else launch_log_rows<false>(mode, rgrid, blk, s, b, N, M, pl.ldk, k1, k0);
@@ -864,7 +1194,20 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
                     b.tapA = step->a ? step->a + (size_t)sw * N : nullptr;
                     b.tapB = step->b ? step->b + (size_t)sw * M : nullptr;
                 }
-                if (LOGV) {
+                if (LOGV && expand) {  // matrix-core passes, no finalize
+                    if (sw == 0)
+                        hipLaunchKernelGGL((sa_mrows_kernel<0>), mrgrid, mblk, 0, s, b, N, M,
+                                           pl.ldk, k1, k0);
+                    else
+                        hipLaunchKernelGGL((sa_mrows_kernel<1>), mrgrid, mblk, 0, s, b, N, M,
+                                           pl.ldk, k1, k0);
+                    if (sw == 0)
+                        hipLaunchKernelGGL((sa_mcols_kernel<true>), mcgrid, mblk, 0, s, b, N, M,
+                                           pl.ldk, k1, k0, last);
+                    else
+                        hipLaunchKernelGGL((sa_mcols_kernel<false>), mcgrid, mblk, 0, s, b, N, M,
+                                           pl.ldk, k1, k0, last);
+                } else if (LOGV) {
                     log_rows(sw == 0 ? 0 : 1);
                     if (expand)
                         hipLaunchKernelGGL((sa_cols_kernel<true, true>), cgrid, blk, 0, s, b, N,
@@ -908,15 +1251,24 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             }
             b.tapA = b.tapB = nullptr;
             b.tapS = step ? step->s : nullptr;
-            if (LOGV)
+            if (LOGV)  // the final pass reads P, v and the weights the column passes left
                 log_rows(2);
             else
                 hipLaunchKernelGGL((sa_rows_kernel<false, 2, kSaFinalRows, kSaFinalWaves>),
                                    dim3((unsigned)finalBlocks), dim3(kWave * kSaFinalWaves), 0, s,
                                    b, N, M, pl.ldk, k1, k0);
             b.tapS = nullptr;
+            // the next inner iteration: its temperature, and whether its
+            // passes run on the matrix cores (then the solve prepares them)
+            const bool more = (i0 + 1 < I0) || (Titer < JMAX);
+            const float Tn = (i0 + 1 < I0) ? T : T * p.TFACTOR;
+            float k1n = 0.f, k0n = 0.f;
+            bool expn = false;
+            if (more) consts(Tn, k1n, k0n, expn);
             hipLaunchKernelGGL(sa_solve_kernel, dim3(1), dim3(kWave * kSaSolveWaves), 0, s, b,
-                               nmom, finBlocks, T, it);
+                               (LOGV && expand) ? mfRows : nmom,
+                               (LOGV && expand) ? mfCols : finBlocks, T, it, M, pl.ldk,
+                               expn ? 1 : 0, k1n, k0n);
             EMSA_HIP_TRY(hipGetLastError());
         }
         T = T * p.TFACTOR;  // softassign.cu:820, float
