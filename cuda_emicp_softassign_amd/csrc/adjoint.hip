@@ -104,44 +104,6 @@ __device__ inline void step_input(const AdjDesc& d, int k, double* R, double* t)
     }
 }
 
-// Full cyclic Jacobi: A (destroyed) -> eigenvalues w, eigenvectors columns of V.
-__device__ inline void jacobi4_full(double A[4][4], double V[4][4], double w[4]) {
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 32; ++sweep) {
-        double off = 0.0, diag = 0.0;
-        for (int i = 0; i < 4; ++i) {
-            diag += A[i][i] * A[i][i];
-            for (int j = i + 1; j < 4; ++j) off += A[i][j] * A[i][j];
-        }
-        if (off <= 1e-36 * diag || off == 0.0) break;
-        for (int p = 0; p < 3; ++p)
-            for (int r = p + 1; r < 4; ++r) {
-                const double apq = A[p][r];
-                if (apq == 0.0) continue;
-                const double tau = (A[r][r] - A[p][p]) / (2.0 * apq);
-                const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
-                for (int k = 0; k < 4; ++k) {
-                    const double akp = A[k][p], akq = A[k][r];
-                    A[k][p] = c * akp - s * akq;
-                    A[k][r] = s * akp + c * akq;
-                }
-                for (int k = 0; k < 4; ++k) {
-                    const double apk = A[p][k], aqk = A[r][k];
-                    A[p][k] = c * apk - s * aqk;
-                    A[r][k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < 4; ++k) {
-                    const double vkp = V[k][p], vkq = V[k][r];
-                    V[k][p] = c * vkp - s * vkq;
-                    V[k][r] = s * vkp + c * vkq;
-                }
-            }
-    }
-    for (int k = 0; k < 4; ++k) w[k] = A[k][k];
-}
-
 // Horn's step backwards (see the file comment). gR (row-major), gt: the
 // gradient of the step's outputs. coef = dL/d of the step's moments in their
 // forward order: [Lambda, SX(3), SY(3), SXY(9)], SXY[i][j] = sum lambda X'_i y_j
@@ -167,13 +129,45 @@ __device__ inline void horn_backward(const double* rec, const double* gR, const 
     A[3][3] = S[8] - S[0] - S[4];
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < i; ++j) A[i][j] = A[j][i];
-    double V[4][4], w[4];
-    jacobi4_full(A, V, w);
-    int top = 0;
-    for (int k = 1; k < 4; ++k)
-        if (w[k] > w[top]) top = k;
+    // q: the top eigenvector of A, recovered from the step's R = R(q) (the
+    // sign of q does not matter: R is quadratic in q and gA below is too)
+    const double* Rs = rec + 17;
     double q[4];
-    for (int i = 0; i < 4; ++i) q[i] = V[i][top];
+    {
+        // the largest of 4 q_i^2 = 1 + {tr, 2 R00 - tr, 2 R11 - tr, 2 R22 - tr}
+        const double tr = Rs[0] + Rs[4] + Rs[8];
+        const double m[4] = {tr, 2.0 * Rs[0] - tr, 2.0 * Rs[4] - tr, 2.0 * Rs[8] - tr};
+        int big = 0;
+        for (int i = 1; i < 4; ++i)
+            if (m[i] > m[big]) big = i;
+        if (big == 0) {
+            const double h = 2.0 * sqrt(1.0 + tr);  // 4 q0
+            q[0] = 0.25 * h;
+            q[1] = (Rs[7] - Rs[5]) / h;
+            q[2] = (Rs[2] - Rs[6]) / h;
+            q[3] = (Rs[3] - Rs[1]) / h;
+        } else if (big == 1) {
+            const double h = 2.0 * sqrt(1.0 + Rs[0] - Rs[4] - Rs[8]);  // 4 qx
+            q[0] = (Rs[7] - Rs[5]) / h;
+            q[1] = 0.25 * h;
+            q[2] = (Rs[1] + Rs[3]) / h;
+            q[3] = (Rs[2] + Rs[6]) / h;
+        } else if (big == 2) {
+            const double h = 2.0 * sqrt(1.0 - Rs[0] + Rs[4] - Rs[8]);  // 4 qy
+            q[0] = (Rs[2] - Rs[6]) / h;
+            q[1] = (Rs[1] + Rs[3]) / h;
+            q[2] = 0.25 * h;
+            q[3] = (Rs[5] + Rs[7]) / h;
+        } else {
+            const double h = 2.0 * sqrt(1.0 - Rs[0] - Rs[4] + Rs[8]);  // 4 qz
+            q[0] = (Rs[3] - Rs[1]) / h;
+            q[1] = (Rs[2] + Rs[6]) / h;
+            q[2] = (Rs[5] + Rs[7]) / h;
+            q[3] = 0.25 * h;
+        }
+        const double nq = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (int i = 0; i < 4; ++i) q[i] /= nq;
+    }
     const double q0 = q[0], qx = q[1], qy = q[2], qz = q[3];
     const double R[9] = {q0 * q0 + qx * qx - qy * qy - qz * qz, 2 * (qx * qy - q0 * qz),
                          2 * (qx * qz + q0 * qy),           2 * (qy * qx + q0 * qz),
@@ -197,18 +191,58 @@ __device__ inline void horn_backward(const double* rec, const double* gR, const 
     gq[3] = 2 * (-qz * G[0] - q0 * G[1] + qx * G[2] + q0 * G[3] - qz * G[4] + qy * G[5] +
                  qx * G[6] + qy * G[7] + qz * G[8]);
     // q = top eigenvector of A: gA = sum_{j != top} (v_j . gq) / (w_top - w_j) v_j q^T
+    // = z q^T with z = (w_top I - A)^+ gq, i.e. z solves
+    // (w_top I - A + s q q^T) z = gq - (q . gq) q (s > 0 lifts the null
+    // direction q; z comes out orthogonal to q), by Gaussian elimination
+    // with partial pivoting. A degenerate top eigenvalue (zero pivot) gives
+    // no gradient through q, as the eigenvector sum skips zero gaps.
     double gA[4][4];
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) gA[a][b] = 0.0;
-    for (int j = 0; j < 4; ++j) {
-        if (j == top) continue;
-        const double gap = w[top] - w[j];
-        if (gap == 0.0) continue;
-        double dot = 0.0;
-        for (int i = 0; i < 4; ++i) dot += V[i][j] * gq[i];
-        const double c = dot / gap;
-        for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 4; ++b) gA[a][b] += c * V[a][j] * q[b];
+    {
+        double Aq[4], wtop = 0.0, qg = 0.0, an = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            Aq[i] = A[i][0] * q[0] + A[i][1] * q[1] + A[i][2] * q[2] + A[i][3] * q[3];
+            wtop += q[i] * Aq[i];
+            qg += q[i] * gq[i];
+            for (int j = 0; j < 4; ++j) an = fmax(an, fabs(A[i][j]));
+        }
+        const double sl = fmax(fabs(wtop), an);  // the lift: of the order of A's eigenvalues
+        double Bm[4][5];
+        for (int i = 0; i < 4; ++i) {
+            for (int j = 0; j < 4; ++j) Bm[i][j] = (i == j ? wtop : 0.0) - A[i][j] + sl * q[i] * q[j];
+            Bm[i][4] = gq[i] - qg * q[i];
+        }
+        bool ok = sl > 0.0;
+        for (int c = 0; c < 4 && ok; ++c) {
+            int pv = c;
+            for (int i = c + 1; i < 4; ++i)
+                if (fabs(Bm[i][c]) > fabs(Bm[pv][c])) pv = i;
+            if (!(fabs(Bm[pv][c]) > 1e-13 * sl)) {
+                ok = false;
+                break;
+            }
+            if (pv != c)
+                for (int j = c; j < 5; ++j) {
+                    const double t = Bm[c][j];
+                    Bm[c][j] = Bm[pv][j];
+                    Bm[pv][j] = t;
+                }
+            for (int i = c + 1; i < 4; ++i) {
+                const double f = Bm[i][c] / Bm[c][c];
+                for (int j = c; j < 5; ++j) Bm[i][j] -= f * Bm[c][j];
+            }
+        }
+        if (ok) {
+            double z[4];
+            for (int i = 3; i >= 0; --i) {
+                double t = Bm[i][4];
+                for (int j = i + 1; j < 4; ++j) t -= Bm[i][j] * z[j];
+                z[i] = t / Bm[i][i];
+            }
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) gA[a][b] = z[a] * q[b];
+        }
     }
     // A = N(S): gS[m] = sum_{ab} gA[a][b] dA_ab / dS_m
     double gS[9];
