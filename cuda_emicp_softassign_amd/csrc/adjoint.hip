@@ -8,10 +8,13 @@
 // lambda_r X'_r). No affinity is stored.
 //
 // Backward, per step from K-1 down to 0, given dL/dR_{k+1}, dL/dt_{k+1}:
-//  1. adj_solve: Horn's step backwards -- t = Xc - R Yc, R = R(q), q the top
-//     eigenvector of N(S) (dq = sum_{j != top} v_j v_j^T dN q / (w_top - w_j)),
-//     S = sum lambda X' y^T - Lambda Xc Yc^T, Xc = SX / Lambda, Yc = SY / Lambda
-//     -- giving the 16 moment coefficients dL/d(Lambda, SX, SY, SXY).
+//  1. Horn's step backwards -- t = Xc - R Yc, R = R(q), q the top
+//     eigenvector of N(S) (dq = sum_{j != top} v_j v_j^T dN q / (w_top - w_j),
+//     evaluated as one 4 x 4 solve), S = sum lambda X' y^T - Lambda Xc Yc^T,
+//     Xc = SX / Lambda, Yc = SY / Lambda -- giving the 16 moment coefficients
+//     dL/d(Lambda, SX, SY, SXY). A kernel of its own (adj_solve) for the last
+//     step; for the others the column-0 thread of the previous step's adj_cols
+//     runs it, so a step is three launches.
 //  2. adj_pairs: per source row, dL/dB, dL/dC, dL/dW from lambda = B / sqrt(C),
 //     lambda X' = W / sqrt(C); then every pair recomputes
 //     e = exp(-|x - p|^2 / (2 sigma^2)) (compensated difference, as the forward)
@@ -24,7 +27,7 @@
 //  4. adj_cols: per target column, dL/dx_c summed over row tiles in a fixed
 //     order (+ the floored rows' 1/N share); its column-0 thread of each
 //     registration also writes the new (dL/dR_k, dL/dt_k) and
-//     dL/dsigma_0^2 += f^k dL/dsigma_k^2.
+//     dL/dsigma_0^2 += f^k dL/dsigma_k^2, then step k-1's stage 1.
 // All reductions have a fixed order; no workgroup waits on another.
 #include "emsa_device.h"
 #include "emsa_internal.h"
@@ -560,7 +563,7 @@ adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_
 // unchanged, so its gradient passes through.
 __global__ void __launch_bounds__(256)
 adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols, int k,
-                double fk, double inv_ka) {
+                double fk, double inv_ka, int next) {
     const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gc >= totalCols) return;
     const int ri = find_by(regs, nregs, (int)gc, [](const AdjDesc& a) { return a.colBase; });
@@ -580,6 +583,10 @@ adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols
             for (int i = 0; i < 12; ++i) d.adj[i] = m[i];  // dL/dR_k, dL/dt_k
         d.adj[12] += fk * m[12];                          // dL/dsigma_0^2
         for (int i = 0; i < 3; ++i) d.adj[13 + i] = gfl[i];
+        // Horn's step backwards for the next (earlier) step: its coefficients
+        // for the pair pass that follows (the pair pass of this step has read
+        // the current ones)
+        if (next >= 0) horn_backward(d.rec + (size_t)next * EMSA_TRACE_W, d.adj, d.adj + 9, d.coef);
     }
     double g[3] = {0.0, 0.0, 0.0};  // the pair terms, ka times dL/dx (see adj_pairs_kernel)
     for (int t = 0; t < d.rowTiles; ++t) {
@@ -781,7 +788,8 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
         // dL/dsigma^2 gets sum ge e s / (2 sigma^4) from the affinities
         const double gsig_scale = -1.0 / (2.0 * s2);
         const double fk = std::pow((double)factor, k);
-        hipLaunchKernelGGL(adj_solve_kernel, dim3(count), dim3(64), 0, s, desc, k);
+        if (k == steps - 1)  // later steps: the previous columns kernel runs it
+            hipLaunchKernelGGL(adj_solve_kernel, dim3(count), dim3(64), 0, s, desc, k);
         if (sig[k] < kCompSigma2)
             hipLaunchKernelGGL(adj_pairs_kernel<true>, dim3((unsigned)tiles),
                                dim3(kWave * kAdjWaves), 0, s, desc, count, k, ka, minv);
@@ -791,7 +799,7 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
         hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)rowTiles), dim3(kAdjRows), 0, s, desc,
                            count, k, gsig_scale, outlier_dsig, inv_ka);
         hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
-                           desc, count, cols, k, fk, inv_ka);
+                           desc, count, cols, k, fk, inv_ka, k - 1);
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(adj_output_kernel, dim3(count), dim3(64), 0, s, desc, dgRt0, dgsigma);
