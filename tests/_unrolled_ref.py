@@ -117,10 +117,11 @@ def softassign_unrolled(X, Y, Rt0, T_0, JMAX, I0, I1, TFACTOR, alpha, moutlier=N
     moutlier None: exp(-1) / sqrt(T_0) (differentiable in T_0). Returns Rt (12,)."""
     R = Rt0[:9].reshape(3, 3)
     t = Rt0[9:]
-    mo = torch.exp(torch.tensor(-1.0, dtype=X.dtype)) / torch.sqrt(T_0) if moutlier is None \
-        else torch.tensor(moutlier, dtype=X.dtype)
-    o_row = mo * torch.ones(X.shape[0], dtype=X.dtype)
-    o_col = mo * torch.ones(Y.shape[0], dtype=X.dtype)
+    kw = dict(dtype=X.dtype, device=X.device)
+    mo = torch.exp(torch.tensor(-1.0, **kw)) / torch.sqrt(T_0) if moutlier is None \
+        else torch.tensor(moutlier, **kw)
+    o_row = mo * torch.ones(X.shape[0], **kw)
+    o_col = mo * torch.ones(Y.shape[0], **kw)
     for j in range(JMAX):
         T = T_0 * TFACTOR ** j
         for _ in range(I0):
