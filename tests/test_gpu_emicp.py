@@ -255,3 +255,45 @@ def test_dropin_cxx_caller():
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "dropin OK" in r.stdout
+
+
+def test_zero_lambda_row_contributes_nothing(E, oracle):
+    """The documented deviation at lambda_r = 0: a source row so far from the
+    target that all its affinities underflow while the outlier term keeps C_r
+    above the 1e-6 floor. The reference (src/emicp.cu:540-549) divides 0 by 0
+    there: Xc and S turn NaN and Horn's step fails; the FP64 oracle and
+    emicp_cpu (the reference's FP32 operations) keep that, so those steps
+    leave (R, t) as they were (emicp_cpu flags them in its trace) until the
+    row is floored at smaller sigma^2. The GPU step gives lambda_r = 0
+    exactly, C_r = the outlier term, and otherwise the oracle's step without
+    that row: the row contributes nothing, and no step fails."""
+    X, Y, Rt, tt = uniform_cube(1007, seed=0)
+    far = np.array([[3.0, 3.0, 3.0]], np.float32)
+    Yf = np.vstack([Y, far]).astype(np.float32)
+    R0 = rotation([1, 2, 3], 0.05) @ Rt
+    t0 = tt + 0.002
+    s2 = 0.01
+    o_far = oracle.emicp_step(X.astype(np.float64), Yf.astype(np.float64), _f32(R0).astype(float),
+                              _f32(t0).astype(float), s2, 0.01)
+    assert np.isnan(o_far["Xc"]).all() and np.isnan(o_far["S"]).any()  # the reference's 0/0
+    np.testing.assert_array_equal(o_far["R"], _f32(R0).astype(float))  # failed solve: R kept
+    o = oracle.emicp_step(X.astype(np.float64), Y.astype(np.float64), _f32(R0).astype(float),
+                          _f32(t0).astype(float), s2, 0.01)
+    g = E.emicp_step(X, Yf, R0, t0, s2, 0.01)
+    assert g["lam"][-1] == 0.0
+    assert abs(g["C"][-1] - np.exp(-0.01 / s2)) < 1e-6
+    np.testing.assert_allclose(g["lam"][:-1], o["lam"], rtol=1e-5)
+    rec = g["record"]
+    assert abs(rec[1] - o["Lambda"]) <= 1e-5 * o["Lambda"]
+    np.testing.assert_allclose(rec[2:5], o["Xc"], atol=1e-6)
+    np.testing.assert_allclose(rec[5:8], o["Yc"], atol=1e-6)
+    assert geodesic(g["R"].astype(np.float64), o["R"]) < 2e-5
+    np.testing.assert_allclose(g["t"], o["t"], atol=2e-6)
+    # the whole registration: no failed step on the GPU; emicp_cpu (FP32, the
+    # reference's order) fails every step before the row is floored
+    R, t, tr = E.emicp_traced(X, Yf, np.eye(3), np.zeros(3))
+    assert np.isfinite(R).all() and np.isfinite(t).all() and (tr[:, 31] == 0).all()
+    _, _, trc = E.emicp_cpu(X, Yf, np.eye(3), np.zeros(3), traced=True)
+    failed = trc[:, 31] != 0
+    assert failed[0] and not failed[-1]
+    assert (np.exp(-0.01 / trc[failed, 0]) > 1e-6).all()  # failed only while C_r is kept up
