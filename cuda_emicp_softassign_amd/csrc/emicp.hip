@@ -38,6 +38,7 @@
 // reproducible run to run.
 #include "emsa_device.h"
 #include "emsa_internal.h"
+#include "emsa_order.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -82,249 +83,27 @@ __device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, in
 // sum stays bit for bit what the full loop over the same order gives. Per-row
 // results are written back in the caller's row order.
 // ---------------------------------------------------------------------------
-constexpr int kOrdThreads = 1024;
-constexpr int kOrdWaves = kOrdThreads / kWave;
-constexpr int kOrdRounds = 4;                        // keys per thread per tile
-constexpr int kOrdKeys = kOrdThreads * kOrdRounds;   // keys per tile
-constexpr int kOrdBits = 10;
 // |exp2 argument| beyond which e is 0 in FP32 (2^-150 already rounds to 0);
 // the margin covers the rounding of the box distance.
 constexpr float kCullArg = 160.f;
 
-// Index of cell (x, y, z) (kOrdBits bits each) along the 3D Hilbert curve:
-// J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707 (2004):
-// the axes are turned into the transposed index, whose bits are then
-// interleaved from the most significant level down.
-__host__ __device__ inline unsigned hilbert_key(unsigned x, unsigned y, unsigned z) {
-    unsigned v[3] = {x, y, z};
-    for (unsigned q = 1u << (kOrdBits - 1); q > 1; q >>= 1) {
-        const unsigned p = q - 1;
-        for (int i = 0; i < 3; ++i) {
-            if (v[i] & q) {
-                v[0] ^= p;
-            } else {
-                const unsigned t = (v[0] ^ v[i]) & p;
-                v[0] ^= t;
-                v[i] ^= t;
-            }
-        }
-    }
-    v[1] ^= v[0];
-    v[2] ^= v[1];
-    unsigned t = 0;
-    for (unsigned q = 1u << (kOrdBits - 1); q > 1; q >>= 1)
-        if (v[2] & q) t ^= q - 1;
-    unsigned key = 0;
-    for (int b = kOrdBits - 1; b >= 0; --b)
-        for (int i = 0; i < 3; ++i) key = (key << 1) | (((v[i] ^ t) >> b) & 1u);
-    return key;
-}
-
 // One workgroup per cloud: blockIdx.x = 2 * registration + (0: target,
 // 1: source); only the registration that owns a pair's buffers orders it.
-// Writes the permutation (working position -> caller's index): the Hilbert
-// key of every point, then a stable least-significant-digit radix sort of
-// (key, index), 8 bits per pass. A pass histograms the digits, then scatters
-// tiles of kOrdKeys keys in order; within a tile, equal digits are ranked
-// inside a wave with ballots (kOrdRounds rounds of 64 consecutive keys) and
-// across waves with per-digit prefix counts in LDS. Non-finite points get the
-// largest key.
+// The permutation (working position -> caller's index), the cloud in working
+// order (the target padded with far sentinels past N), and the target's
+// column-group boxes.
 __global__ void __launch_bounds__(kOrdThreads)
 emicp_order_kernel(const EmRegDesc* __restrict__ regs) {
     const EmRegDesc& d = regs[blockIdx.x >> 1];
     if (!d.padOwner || !d.rowperm) return;  // workgroup-uniform
     const bool src = (blockIdx.x & 1) != 0;
     const int n = src ? d.M : d.N;
-    const float* __restrict__ P = src ? d.Y : d.X;
-    int* __restrict__ perm = src ? d.rowperm : d.colperm;
-    unsigned* kA = d.sortTmp + (src ? 4 * (size_t)d.N : 0);
-    unsigned* vA = kA + n;
-    unsigned* kB = vA + n;
-    unsigned* vB = kB + n;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-
-    __shared__ float boxs[kOrdWaves][6];
-    __shared__ unsigned base[256], tot[256], wsum[4];
-    __shared__ unsigned wcnt[kOrdWaves * 256];
-
-    // bounding box of the finite points
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int i = tid; i < n; i += kOrdThreads)
-        for (int a = 0; a < 3; ++a) {
-            const float v = P[(size_t)a * n + i];
-            if (fabsf(v) <= FLT_MAX) {
-                lo[a] = fminf(lo[a], v);
-                hi[a] = fmaxf(hi[a], v);
-            }
-        }
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min(lo[a]);
-        hi[a] = wave_max(hi[a]);
-    }
-    if (lane == 0)
-        for (int a = 0; a < 3; ++a) {
-            boxs[wave][a] = lo[a];
-            boxs[wave][3 + a] = hi[a];
-        }
-    __syncthreads();
-    float scale[3];
-    for (int a = 0; a < 3; ++a) {
-        for (int w = 0; w < kOrdWaves; ++w) {
-            lo[a] = fminf(lo[a], boxs[w][a]);
-            hi[a] = fmaxf(hi[a], boxs[w][3 + a]);
-        }
-        scale[a] = (hi[a] > lo[a]) ? (float)((1 << kOrdBits) - 1) / (hi[a] - lo[a]) : 0.f;
-    }
-    for (int i = tid; i < n; i += kOrdThreads) {
-        unsigned q[3];
-        bool fin = true;
-        for (int a = 0; a < 3; ++a) {
-            const float v = P[(size_t)a * n + i];
-            fin = fin && fabsf(v) <= FLT_MAX;
-            // NaN from an infinite range times a zero scale clamps to 0
-            const float f = fminf(fmaxf((v - lo[a]) * scale[a], 0.f), (float)((1 << kOrdBits) - 1));
-            q[a] = (unsigned)f;
-        }
-        kA[i] = fin ? hilbert_key(q[0], q[1], q[2]) : 0xFFFFFFFFu;
-        vA[i] = (unsigned)i;
-    }
-    __syncthreads();
-
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int sh = 8 * pass;
-        const unsigned* ki = (pass & 1) ? kB : kA;
-        const unsigned* vi = (pass & 1) ? vB : vA;
-        unsigned* ko = (pass & 1) ? kA : kB;
-        unsigned* vo = (pass & 1) ? vA : vB;
-        if (tid < 256) tot[tid] = 0;
-        __syncthreads();
-        for (int i = tid; i < n; i += kOrdThreads) atomicAdd(&tot[(ki[i] >> sh) & 255u], 1u);
-        __syncthreads();
-        {  // exclusive scan of the 256 digit counts by waves 0..3
-            unsigned h = 0, x = 0;
-            if (tid < 256) {
-                h = tot[tid];
-                x = h;
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const unsigned y = __shfl_up(x, o, kWave);
-                    if (lane >= o) x += y;
-                }
-                if (lane == kWave - 1) wsum[wave] = x;
-            }
-            __syncthreads();
-            if (tid < 256) {
-                unsigned off = 0;
-                for (int w = 0; w < wave; ++w) off += wsum[w];
-                base[tid] = off + x - h;
-            }
-        }
-        // tiles of kOrdKeys keys; wave w ranks keys t0 + 256 w + 64 j + lane in
-        // rounds j = 0..3 (index order), keeping its running count per digit
-        // in wcnt (all lanes read it before the digit's first lane adds the
-        // round's count: one wave's LDS instructions execute in order)
-        for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
-        __syncthreads();
-        for (int t0 = 0; t0 < n; t0 += kOrdKeys) {
-            unsigned key[kOrdRounds], val[kOrdRounds], rank[kOrdRounds];
-#pragma unroll
-            for (int j = 0; j < kOrdRounds; ++j) {
-                const int i = t0 + kWave * (kOrdRounds * wave + j) + lane;
-                const bool valid = i < n;
-                key[j] = valid ? ki[i] : 0u;
-                val[j] = valid ? vi[i] : 0u;
-                const unsigned dig = (key[j] >> sh) & 255u;
-                // lanes of this round holding the same digit
-                unsigned long long same = __ballot(valid);
-                for (int b = 0; b < 8; ++b) {
-                    const bool bit = ((dig >> b) & 1u) != 0;
-                    const unsigned long long bb = __ballot(bit);
-                    same &= bit ? bb : ~bb;
-                }
-                const unsigned r = (unsigned)__popcll(same & below);
-                const unsigned pre = valid ? wcnt[wave * 256 + dig] : 0u;
-                wave_lds_sync();
-                if (valid && r == 0) wcnt[wave * 256 + dig] = pre + (unsigned)__popcll(same);
-                wave_lds_sync();
-                rank[j] = pre + r;
-            }
-            __syncthreads();
-            if (tid < 256) {  // per digit: offsets of the waves, then the tile total
-                unsigned run = 0;
-                for (int w = 0; w < kOrdWaves; ++w) {
-                    const unsigned c = wcnt[w * 256 + tid];
-                    wcnt[w * 256 + tid] = run;
-                    run += c;
-                }
-                tot[tid] = run;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kOrdRounds; ++j) {
-                const int i = t0 + kWave * (kOrdRounds * wave + j) + lane;
-                if (i < n) {
-                    const unsigned dig = (key[j] >> sh) & 255u;
-                    const unsigned pos = base[dig] + wcnt[wave * 256 + dig] + rank[j];
-                    ko[pos] = key[j];
-                    vo[pos] = val[j];
-                }
-            }
-            __syncthreads();
-            if (tid < 256) base[tid] += tot[tid];
-            for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
-            __syncthreads();
-        }
-        __syncthreads();  // this pass's output is the next pass's input
-    }
-    for (int i = tid; i < n; i += kOrdThreads) perm[i] = (int)vA[i];
-
-    // the cloud in working order: the padded target (far sentinels past N) or
-    // the source copy; 4 points per thread in flight
-    const int ld = src ? n : d.ldx;
-    float* __restrict__ W = src ? d.Ys : d.Xpad;
-    for (int i0 = 0; i0 < ld; i0 += 4 * kOrdThreads) {
-        float v[4][3];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * kOrdThreads + tid;
-            const int o = (i < n) ? (int)vA[i] : -1;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) v[u][a] = (o >= 0) ? P[(size_t)a * n + o] : kSentinel;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * kOrdThreads + tid;
-            if (i < ld)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) W[(size_t)a * ld + i] = v[u][a];
-        }
-    }
+    order_cloud(src ? d.Y : d.X, n, src ? d.rowperm : d.colperm,
+                d.sortTmp + (src ? 4 * (size_t)d.N : 0), src ? d.Ys : d.Xpad, src ? n : d.ldx,
+                kSentinel);
     if (src || !d.gbox) return;
     __syncthreads();  // the boxes read the padded target written above
-    for (int g = tid; g < d.ldx / kCullCols; g += kOrdThreads) {
-        // empty group (padding only): lo > hi, always skipped; a group with a
-        // non-finite point: the whole space, never skipped
-        float lo3[3] = {INFINITY, INFINITY, INFINITY}, hi3[3] = {-INFINITY, -INFINITY, -INFINITY};
-        bool fin = true;
-        const int cn = min(kCullCols, n - g * kCullCols);
-        for (int a = 0; a < 3; ++a) {
-            const float4* q = reinterpret_cast<const float4*>(d.Xpad + (size_t)a * d.ldx + g * kCullCols);
-            const float4 q0 = q[0], q1 = q[1];
-            const float c8[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-            for (int u = 0; u < kCullCols; ++u)
-                if (u < cn) {
-                    fin = fin && fabsf(c8[u]) <= FLT_MAX;
-                    lo3[a] = fminf(lo3[a], c8[u]);
-                    hi3[a] = fmaxf(hi3[a], c8[u]);
-                }
-        }
-        float* b = d.gbox + (size_t)g * 8;
-        *reinterpret_cast<float4*>(b) =
-            make_float4(fin ? lo3[0] : -INFINITY, fin ? lo3[1] : -INFINITY, fin ? lo3[2] : -INFINITY, 0.f);
-        *reinterpret_cast<float4*>(b + 4) =
-            make_float4(fin ? hi3[0] : INFINITY, fin ? hi3[1] : INFINITY, fin ? hi3[2] : INFINITY, 0.f);
-    }
+    group_boxes<kCullCols>(d.Xpad, d.ldx, n, d.gbox);
 }
 
 // One workgroup per registration: sum the target in FP64 (for floored rows,

@@ -161,9 +161,10 @@ struct DeviceCtx {
     // softassign workspaces (sastep: staging of emsa_softassign_step's slack
     // vectors and taps)
     DevBuf K, u, v, orow, ocol, colpart, samom, sastate, sastep, sapack;
-    // ICP workspaces: per-tile nearest candidates, row-block moments, state,
-    // per-row neighbour and distance (the correspondence query)
-    DevBuf icpDist, icpIdx, icpMom, icpState, icpRowIdx, icpRowDist;
+    // ICP workspaces: the ordered clouds, group boxes and per-row candidates
+    // (O(N + M)), row-block moments, state, per-row neighbour and distance
+    // (the correspondence query)
+    DevBuf icpWork, icpMom, icpState, icpRowIdx, icpRowDist;
     // captured EM-ICP launch sequences of the device entry points, replayed
     // when a call repeats the same buffers, sizes and parameters; keys seen
     // once (a sequence is recorded on its second call)

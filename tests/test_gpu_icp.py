@@ -90,6 +90,34 @@ def test_nearest_10k(E, O):
     _check_nearest(E, O, X, Y, R, t)
 
 
+def test_nearest_ties_take_lowest_index(E, O):
+    """Exact duplicates at scattered indices: equal FP32 distances, so each
+    row's neighbour is the lowest index among the copies of its nearest point
+    (the reference's strict "<" scan over the target in order)."""
+    rng = np.random.default_rng(11)
+    base = rng.uniform(-1, 1, (700, 3)).astype(np.float32)
+    X = np.concatenate([base, base[rng.permutation(700)], base[:300]])
+    X = X[rng.permutation(len(X))]
+    Y = rng.uniform(-1, 1, (900, 3)).astype(np.float32)
+    idx, d2 = E.icp_nearest(X, Y, np.eye(3), np.zeros(3))
+    same = (X[None, :, :] == X[idx][:, None, :]).all(2)  # copies of each row's neighbour
+    first = same.argmax(1)
+    assert np.array_equal(idx, first)
+    _check_nearest(E, O, X, Y, np.eye(3), np.zeros(3))
+
+
+@pytest.mark.parametrize("shift", [0.0, 8.0])
+def test_nearest_clustered_and_far(E, O, shift):
+    """Dense clusters with empty space between them, and a source moved far
+    off the target: the candidate bounds have to prune and to keep the rows
+    whose nearest lies in another cluster."""
+    rng = np.random.default_rng(12)
+    centres = rng.uniform(-1, 1, (12, 3))
+    X = (centres[rng.integers(0, 12, 3000)] + rng.normal(0, 0.01, (3000, 3))).astype(np.float32)
+    Y = rng.uniform(-1.2, 1.2, (1200, 3)).astype(np.float32)
+    _check_nearest(E, O, X, Y, random_rotation(rng, 30.0), np.full(3, shift))
+
+
 @pytest.mark.parametrize("seed,n,m", [(5, 1007, 1007), (6, 2000, 1500), (7, 700, 1200)])
 def test_icp_matches_oracle(E, O, seed, n, m):
     X, Y, Rg, tg = _pair(n, m, seed=seed)
