@@ -44,6 +44,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace emsa {
 
@@ -466,12 +467,15 @@ constexpr int kMfPiece = 256;  // columns staged per wave at a time
 // row stride of a wave's stage: 4 floats of padding put the four component
 // rows, which lanes 4b + j read together (row j), in different LDS banks
 constexpr int kMfStageLd = kMfPiece + 4;
+// EXP stage value of a slot the piece does not fill: exp2 of its exponent is 0
+// (-|w|^2 overflows to -inf, and the products with |u|^2 <= xmax stay finite)
+constexpr float kExpFar = 1.0e30f;
 
 template <bool COMP, int UNR, int NACC, bool FUSE>
 __global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
 emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
-                           unsigned long long* __restrict__ visits, double outlier, int last,
-                           int iter) {
+                           float xmax, unsigned long long* __restrict__ visits, double outlier,
+                           int last, int iter) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
@@ -508,39 +512,98 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     const cfloat_p Xy = (cfloat_p)(d.Xpad + ldx);
     const cfloat_p Xz = (cfloat_p)(d.Xpad + 2 * (size_t)ldx);
 
-    // box of the wave's valid rows; a non-finite row (NaN affinities in every
-    // column) turns the skipping off for the wave
+    // box of the wave's valid rows (for the skipping and the expanded form's
+    // test); a non-finite row (NaN affinities in every column) turns both off
+    // for the wave
     const bool rv = r < M;
     const float nk = ka * ka;
     bool doCull = cull && d.gbox;
     float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
-    if (doCull) {
+    bool finiteRows = false;
+    if (doCull || xmax > 0.f) {
         const bool bad = rv && !(fabsf(px) <= FLT_MAX && fabsf(py) <= FLT_MAX && fabsf(pz) <= FLT_MAX);
-        doCull = __ballot(bad) == 0;
+        finiteRows = __ballot(bad) == 0;
         const float pv[3] = {px, py, pz};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            rlo[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
-                __float_as_int(wave_min(rv ? pv[a] : INFINITY))));
-            rhi[a] = __int_as_float(__builtin_amdgcn_readfirstlane(
-                __float_as_int(wave_max(rv ? pv[a] : -INFINITY))));
+            rlo[a] = lane63(wave_min_last(rv ? pv[a] : INFINITY));
+            rhi[a] = lane63(wave_max_last(rv ? pv[a] : -INFINITY));
+        }
+    }
+    doCull = doCull && finiteRows;
+
+    // Expanded exponent on the matrix cores (EXP), per wave about the centre
+    // c of its rows' box. With w = ka x - ka c (the column, one FMA per
+    // coordinate, staged) and u = ka p - ka c (the row, FP64 then rounded),
+    //   -|d'|^2 = -|u|^2 + 2 w.u - |w|^2,
+    // a v_mfma_f32_16x16x4f32 per 16 rows (s) and 16 columns: A[i][k] (lane
+    // i + 16k) = (w_x, w_y, w_z, 1) of column i, read from the stage, B[k][jr]
+    // (lane jr + 16k) = (2u_x, 2u_y, 2u_z, -|u|^2) of row 16s + jr, and the
+    // accumulator init -|w|^2 of the lane's 4 columns (shared by the four
+    // row groups), so that lane jr + 16q receives the exponents of row
+    // 16s + jr and columns 4q .. 4q + 3. B and W then go through v_mfma_f32_4x4x1f32 as in the
+    // difference form, with (w, 1) as the column operand: lane 4b + j sums
+    // component j over rows 16s + 4(b mod 4) .. + 3 for the columns 4q + v of
+    // phase q = b / 4; the four phases are added at the end, which leaves the
+    // same layout as the difference form, and W = (sum e w + ka c B) / ka.
+    // Per 64 pairs: 1/4 of a 16x16x4 product (instead of 6 VALU ops of the
+    // difference, 9 with the compensation). The rounding of the expanded
+    // terms grows with ka^2 (|w| + |u|)^2, so a wave takes this form only
+    // when all its valid rows lie within a of c with ka^2 a^2 <= xmax
+    // (xmax = 0: never); a wave with a non-finite row keeps the difference.
+    const int j = lane & 3;  // this lane's component in the matrix product
+    bool expd = false;
+    float cwx = 0.f, cwy = 0.f, cwz = 0.f;  // ka c, FP32
+    float bop[4] = {0.f, 0.f, 0.f, 0.f};    // B operand of rows 16s + (lane & 15)
+    if (xmax > 0.f && finiteRows) {
+        float cw[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cw[a] = (rlo[a] <= rhi[a]) ? 0.5f * (rlo[a] + rhi[a]) : 0.f;
+        const float ax = px - cw[0], ay = py - cw[1], az = pz - cw[2];
+        const float a2 = fmaf(ax, ax, fmaf(ay, ay, az * az));
+        const bool bad = rv && !(nk * a2 <= xmax);
+        expd = __ballot(bad) == 0;
+        if (expd) {  // wave-uniform
+            cwx = (float)(kad * (double)cw[0]);
+            cwy = (float)(kad * (double)cw[1]);
+            cwz = (float)(kad * (double)cw[2]);
+            float ux = 0.f, uy = 0.f, uz = 0.f, n2 = 0.f;
+            if (rv) {
+                ux = (float)(qxd - (double)cwx);
+                uy = (float)(qyd - (double)cwy);
+                uz = (float)(qzd - (double)cwz);
+                n2 = -(float)((double)ux * ux + (double)uy * uy + (double)uz * uz);
+            }
+            const int k = lane >> 4;
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) {
+                const int src = 16 * sb + (lane & 15);
+                const float vx = __shfl(2.f * ux, src, kWave), vy = __shfl(2.f * uy, src, kWave),
+                            vz = __shfl(2.f * uz, src, kWave), vn = __shfl(n2, src, kWave);
+                bop[sb] = (k == 0) ? vx : (k == 1) ? vy : (k == 2) ? vz : vn;
+            }
         }
     }
 
     __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfStageLd];
     float (*st)[kMfStageLd] = stage[wave];
-    const int j = lane & 3;  // this lane's component in the matrix product
     // the operand's constant fourth component (B = sum e * 1): the compensated
     // instance writes it once (restaging it per piece kept a constant float4
     // live across the loop and spilled it at 8 waves per SIMD); the plain one
     // restages it with each piece (same-process A/B: writing it once changed
     // its register allocation and made its launches 7% slower)
     static_assert(kMfPiece == 4 * kWave, "one float4 of ones per lane");
-    if (COMP) *reinterpret_cast<float4*>(&st[3][4 * lane]) = make_float4(1.f, 1.f, 1.f, 1.f);
+    *reinterpret_cast<float4*>(&st[3][4 * lane]) = make_float4(1.f, 1.f, 1.f, 1.f);
     float C = 0.f;
     f4 D[NACC];
 #pragma unroll
     for (int a = 0; a < NACC; ++a) D[a] = f4{0.f, 0.f, 0.f, 0.f};
+    // EXP: per 16 rows s, lane jr + 16q sums e^2 of row 16s + jr (Cx) and
+    // the B/W matrix accumulators of phase q (Dx)
+    float Cx[4] = {0.f, 0.f, 0.f, 0.f};
+    f4 Dx[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) Dx[a] = f4{0.f, 0.f, 0.f, 0.f};
     int nvis = 0;  // valid columns visited by this wave
     // The chunk's groups of kCullCols columns are dealt to the 4 waves in
     // turn (group G0 + wave + 4k), so that the columns near the row block,
@@ -550,6 +613,9 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     const int G0 = g * kEmWaves * gpw;
     const int Gend = min(G0 + kEmWaves * gpw, ldx / kCullCols);
     const int nw = (Gend - G0 - wave + kEmWaves - 1) / kEmWaves;  // this wave's groups
+    // the column loop, once per form of the exponent (a wave-uniform choice)
+    auto columns = [&](auto expTag) {
+    constexpr bool EXP = decltype(expTag)::value;
     for (int k0 = 0; k0 < nw; k0 += kPieceGroups) {
         const int ng = min(kPieceGroups, nw - k0);
         const int gbase = G0 + wave + kEmWaves * k0;  // group of piece slot jj: gbase + 4 jj
@@ -569,23 +635,77 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
             keep = (unsigned)__ballot(kg);
             if (keep == 0) continue;
         }
-        // stage [x; y; z; 1] of the piece: lane 2 jj + h holds columns 4h .. 4h+3
-        // of slot jj
+        // stage [x; y; z; 1] of the piece (EXP: [w; 1], and slots past the
+        // piece's groups far away, since a 16-column step may read one): lane
+        // 2 jj + h holds columns 4h .. 4h+3 of slot jj
         {
             const int jl = lane >> 1, h = lane & 1;
+            const int sc = jl * kCullCols + 4 * h;
             if (jl < ng) {
                 const int col = (gbase + kEmWaves * jl) * kCullCols + 4 * h;
-                const int sc = jl * kCullCols + 4 * h;
-                *reinterpret_cast<float4*>(&st[0][sc]) =
-                    *reinterpret_cast<const float4*>(d.Xpad + col);
-                *reinterpret_cast<float4*>(&st[1][sc]) =
-                    *reinterpret_cast<const float4*>(d.Xpad + ldx + col);
-                *reinterpret_cast<float4*>(&st[2][sc]) =
-                    *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + col);
-                if (!COMP) *reinterpret_cast<float4*>(&st[3][sc]) = make_float4(1.f, 1.f, 1.f, 1.f);
+                float4 vx = *reinterpret_cast<const float4*>(d.Xpad + col);
+                float4 vy = *reinterpret_cast<const float4*>(d.Xpad + ldx + col);
+                float4 vz = *reinterpret_cast<const float4*>(d.Xpad + 2 * (size_t)ldx + col);
+                if constexpr (EXP) {
+                    vx = make_float4(fmaf(vx.x, ka, -cwx), fmaf(vx.y, ka, -cwx),
+                                     fmaf(vx.z, ka, -cwx), fmaf(vx.w, ka, -cwx));
+                    vy = make_float4(fmaf(vy.x, ka, -cwy), fmaf(vy.y, ka, -cwy),
+                                     fmaf(vy.z, ka, -cwy), fmaf(vy.w, ka, -cwy));
+                    vz = make_float4(fmaf(vz.x, ka, -cwz), fmaf(vz.y, ka, -cwz),
+                                     fmaf(vz.z, ka, -cwz), fmaf(vz.w, ka, -cwz));
+                }
+                *reinterpret_cast<float4*>(&st[0][sc]) = vx;
+                *reinterpret_cast<float4*>(&st[1][sc]) = vy;
+                *reinterpret_cast<float4*>(&st[2][sc]) = vz;
+            } else if (EXP) {
+                const float4 far = make_float4(kExpFar, kExpFar, kExpFar, kExpFar);
+                *reinterpret_cast<float4*>(&st[0][sc]) = far;
+                *reinterpret_cast<float4*>(&st[1][sc]) = far;
+                *reinterpret_cast<float4*>(&st[2][sc]) = far;
             }
         }
         wave_lds_sync();
+        if constexpr (EXP) {
+            // 16-column steps: slots 2m and 2m + 1 together (a slot the box
+            // test rejected has every e exactly 0 in this form too)
+            unsigned km = (keep | (keep >> 1)) & 0x55555555u;
+            while (km) {  // wave-uniform
+                const int jj = __builtin_ctz(km);
+                km &= km - 1u;
+                const int c = jj * kCullCols;  // in the stage
+                if (visits) {
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int cc0 = (gbase + kEmWaves * (jj + t)) * kCullCols;
+                        if (jj + t < ng) nvis += max(0, min(kCullCols, d.N - cc0));
+                    }
+                }
+                // A: component (lane >> 4) of column c + (lane & 15); the
+                // lane's 4 columns of phase q = lane >> 4: their (w, 1) for
+                // the B/W operand and -|w|^2 for the accumulator init
+                const float aop = st[lane >> 4][c + (lane & 15)];
+                const int cq = c + 4 * (lane >> 4);
+                const f4 xq = *reinterpret_cast<const f4*>(&st[j][cq]);
+                const f4 qx = *reinterpret_cast<const f4*>(&st[0][cq]);
+                const f4 qy = *reinterpret_cast<const f4*>(&st[1][cq]);
+                const f4 qz = *reinterpret_cast<const f4*>(&st[2][cq]);
+                f4 nw2;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) nw2[v] = -fmaf(qx[v], qx[v], fmaf(qy[v], qy[v], qz[v] * qz[v]));
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) {
+                    const f4 ex = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[sb], nw2, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const float e = __builtin_amdgcn_exp2f(ex[v]);
+                        Cx[sb] = fmaf(e, e, Cx[sb]);
+                        Dx[sb] = __builtin_amdgcn_mfma_f32_4x4x1f32(e, xq[v], Dx[sb], 0, 0, 0);
+                    }
+                }
+            }
+            wave_lds_sync();
+            continue;
+        }
         while (keep) {  // wave-uniform: the slots still to visit
             const int jj = __builtin_ctz(keep);
             keep &= keep - 1u;
@@ -626,6 +746,28 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         }
         wave_lds_sync();
     }
+    };
+    if (expd) {
+        columns(std::true_type{});
+        // add the four phases (lanes l, l ^ 16, l ^ 32, l ^ 48, fixed order)
+        // and keep the rows of this lane's 16: C of row `lane`, D of rows
+        // (lane & ~3) .. + 3 as in the difference form
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            Cx[sb] += __shfl_xor(Cx[sb], 16, kWave);
+            Cx[sb] += __shfl_xor(Cx[sb], 32, kWave);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                Dx[sb][i] += __shfl_xor(Dx[sb][i], 16, kWave);
+                Dx[sb][i] += __shfl_xor(Dx[sb][i], 32, kWave);
+            }
+        }
+        const int sl = lane >> 4;
+        C = (sl == 0) ? Cx[0] : (sl == 1) ? Cx[1] : (sl == 2) ? Cx[2] : Cx[3];
+        D[0] = (sl == 0) ? Dx[0] : (sl == 1) ? Dx[1] : (sl == 2) ? Dx[2] : Dx[3];
+    } else {
+        columns(std::false_type{});
+    }
 #pragma unroll
     for (int a = 1; a < NACC; ++a) D[0] += D[a];
     if (visits && lane == 0 && nvis > 0)
@@ -643,6 +785,14 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     const int rb4 = lane & ~3;
 #pragma unroll
     for (int i = 0; i < 4; ++i) red[q * kEmRows + rb4 + i] = D[0][i];
+    if (expd) {  // sums of e w -> W = sum e x = (sum e w + ka c B) / ka
+        wave_lds_sync();
+        const float Bs = red[1 * kEmRows + lane];
+        const float cv[3] = {cwx, cwy, cwz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            red[(2 + a) * kEmRows + lane] = fmaf(cv[a], Bs, red[(2 + a) * kEmRows + lane]) / ka;
+    }
     __syncthreads();
     if (wave == 0) {
         float acc[5];
@@ -724,7 +874,7 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
         const int last = fuse ? fuse->last : 0, iter = fuse ? fuse->iter : 0;
 #define EMSA_AFF(C_, F_)                                                                     \
     hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 4, F_>), dim3(tiles), blk, 0, s, desc, \
-                       nregs, ka, ctx.emCull, visits, outlier, last, iter)
+                       nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter)
         if (fuse) {
             if (comp) EMSA_AFF(true, true);
             else EMSA_AFF(false, true);

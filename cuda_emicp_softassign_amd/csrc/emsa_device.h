@@ -51,6 +51,37 @@ __device__ inline float wave_sum_last(float v) {
     return v;
 }
 
+// Minimum / maximum over the 64 lanes of a wave with DPP-modified operations
+// only, in the pattern of wave_sum_last; valid in lane 63. Lanes a DPP
+// operation does not write keep their value (the `old` operand is v).
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_src(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL,
+                                                      ROW_MASK, 0xF, false));
+}
+__device__ inline float wave_min_last(float v) {
+    v = fminf(v, dpp_src<0xB1, 0xF>(v));
+    v = fminf(v, dpp_src<0x4E, 0xF>(v));
+    v = fminf(v, dpp_src<0x141, 0xF>(v));
+    v = fminf(v, dpp_src<0x140, 0xF>(v));
+    v = fminf(v, dpp_src<0x142, 0xA>(v));
+    v = fminf(v, dpp_src<0x143, 0xC>(v));
+    return v;
+}
+__device__ inline float wave_max_last(float v) {
+    v = fmaxf(v, dpp_src<0xB1, 0xF>(v));
+    v = fmaxf(v, dpp_src<0x4E, 0xF>(v));
+    v = fmaxf(v, dpp_src<0x141, 0xF>(v));
+    v = fmaxf(v, dpp_src<0x140, 0xF>(v));
+    v = fmaxf(v, dpp_src<0x142, 0xA>(v));
+    v = fmaxf(v, dpp_src<0x143, 0xC>(v));
+    return v;
+}
+// The value of lane 63, wave-uniform (an SGPR).
+__device__ inline float lane63(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Block-wide sum of K doubles per thread for a block of `nwaves` waves.
 // lds must hold nwaves*K doubles. Thread k < K of the block receives total k
 // in out (other threads: unspecified). Fixed reduction order: deterministic.

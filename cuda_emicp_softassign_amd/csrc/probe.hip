@@ -168,6 +168,19 @@ extern "C" int emsa_probe_set_affinity_cull(int enable) {
     return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
 }
 
+extern "C" int emsa_probe_set_affinity_expanded(float max_ka2) {
+    clear_error();
+    if (!(max_ka2 >= 0.f) || !(max_ka2 < 1e30f))
+        return set_error("probe_set_affinity_expanded: bound must be finite and >= 0"), EMSA_EINVAL;
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->emExpanded = max_ka2;
+    }
+    return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
+}
+
 extern "C" int emsa_probe_set_row_fuse(int enable) {
     clear_error();
     DeviceCtx* ctx = current_ctx();

@@ -65,6 +65,15 @@ int emsa_probe_hilbert_order(const float *dP, int n, int *dPerm);
  * For A/B measurement; drops the recorded launch sequences of the device. */
 int emsa_probe_set_affinity_cull(int enable);
 
+/* Matrix-core EM-ICP affinity of the current device: a wave of 64 source rows
+ * forms its exponents -|x - p|^2 / (2 sigma^2) on the matrix cores, expanded
+ * about the centre of each block of 4 rows, when every block's rows lie
+ * within a of their centre with (log2 e / (2 sigma^2)) a^2 <= max_ka2, and as
+ * a difference on the VALU otherwise. 0 always takes the difference; the
+ * default is 32. The two forms differ by FP32 rounding only. For A/B
+ * measurement; drops the recorded launch sequences of the device. */
+int emsa_probe_set_affinity_expanded(float max_ka2);
+
 /* Selects, on the current device, where the EM-ICP row finalize runs when
  * every tile of a launch spans its registration's whole target (one column
  * chunk, e.g. emsa_set_tiling(0, tile_cols) with tile_cols >= N): 1 (default)

@@ -71,6 +71,40 @@ def test_step_normalisers_match_oracle(E, oracle, sigma_p2):
     np.testing.assert_allclose(g["t"], o["t"], atol=2e-6)
 
 
+@pytest.mark.parametrize("sigma_p2", [0.01, 4e-3, 2e-3])
+def test_step_normalisers_expanded_form(E, oracle, sigma_p2):
+    """The affinity's exponent expanded about each wave's row centre on the
+    matrix cores (taken by a wave when ka^2 a^2 <= 32, a its rows' distance
+    from the centre, ka^2 = log2 e / (2 sigma^2)) and the VALU difference
+    (bound 0) each give per-row C_r and lambda_r within 1e-5 relative of the
+    FP64 oracle, on a 4000-point cube where every wave qualifies at 1e-2 and
+    some at 2e-3; the two forms differ (the expansion was taken)."""
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    X, Y, Rt, tt = uniform_cube(4000, seed=7)
+    R0 = rotation([1, 2, 3], 0.02) @ Rt
+    t0 = tt + 0.001
+    o = oracle.emicp_step(X.astype(np.float64), Y.astype(np.float64), _f32(R0).astype(float),
+                          _f32(t0).astype(float), sigma_p2, 0.01)
+    keep = o["C"] > 2e-6
+    got = {}
+    try:
+        for bound in (0.0, 32.0):
+            _lib.check("expanded", L.emsa_probe_set_affinity_expanded(bound))
+            g = E.emicp_step(X, Y, R0, t0, sigma_p2, 0.01)
+            relC = np.max(np.abs(g["C"][keep] - o["C"][keep]) / o["C"][keep])
+            relL = np.max(np.abs(g["lam"] - o["lam"]) / np.maximum(o["lam"], 1e-300))
+            print("sigma^2 %g bound %g: rel err C %.1e lambda %.1e" % (sigma_p2, bound, relC, relL),
+                  flush=True)
+            assert relC <= 1e-5 and relL <= 1e-5
+            assert abs(g["record"][1] - o["Lambda"]) <= 1e-5 * o["Lambda"]
+            assert geodesic(g["R"].astype(np.float64), o["R"]) < 2e-5
+            got[bound] = g["C"]
+    finally:
+        _lib.check("expanded", L.emsa_probe_set_affinity_expanded(32.0))
+    assert not np.array_equal(got[0.0], got[32.0])
+
+
 @pytest.mark.parametrize("k", [0, 13, 24, 30, 37])
 def test_step_normalisers_partial_overlap(E, oracle, k):
     """A partial-overlap range-scan pair with few targets (60 x 1500): most
