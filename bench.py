@@ -29,13 +29,17 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 1 (B), 6 (W fma)
-# issued per pair on the VALU at sigma^2 < 2e-3: 3 fma + 3 sub (the scaled,
-# compensated difference), 3 (d^2), 1 (C fma); the exponent scale rides on the
-# coordinates and the sign on the exp's source modifier; B and W go to the
-# matrix cores (one v_mfma_f32_4x4x1f32
-# per column and wave, i.e. per 64 pairs)
+# Instruction mix per 64 evaluated pairs (one wave instruction each) of the
+# affinity kernel's three forms, from its gfx950 assembly and the counters:
+# difference form at sigma^2 < 2e-3: 3 fma + 3 sub (the scaled, compensated
+# difference), 3 (d^2), 1 (C fma) on the VALU, 1 v_exp_f32 and one
+# v_mfma_f32_4x4x1f32 (B and W over the column dimension); plain difference
+# (sigma^2 >= 2e-3): 7 VALU; expanded form: 1/4 v_mfma_f32_16x16x4f32 (the
+# exponents of 16 rows x 16 columns), the same exp and 4x4x1 product, C and
+# the columns' -|w|^2 (2 VALU).
 VALU_PER_PAIR = 10
 VALU_PER_PAIR_PLAIN = 7        # sigma^2 >= 2e-3: plain difference
+VALU_PER_PAIR_EXPANDED = 2
 COMP_BELOW_SIGMA2 = 2e-3
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
@@ -586,26 +590,32 @@ def main():
     # device), so the figure covers the skipping at every sigma^2
     seq_us, seq_pairs, seq_n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
     cp = p.to_c()
-    steps_buf = (ctypes.c_double * (3 * n_sigma))()
+    steps_buf = (ctypes.c_double * (4 * n_sigma))()
     _lib.check("probe_sequence", _lib.load().emsa_probe_emicp_sequence(
         arr, B, 1, ctypes.byref(cp), ctypes.byref(seq_us), ctypes.byref(seq_pairs),
         ctypes.byref(seq_n), steps_buf, n_sigma))
-    per_step = np.frombuffer(steps_buf, dtype=np.float64).reshape(n_sigma, 3)
+    per_step = np.frombuffer(steps_buf, dtype=np.float64).reshape(n_sigma, 4)
     us = ctypes.c_double(seq_us.value / seq_n.value)
     npairs = ctypes.c_double(seq_pairs.value / seq_n.value)
     pe, pf = ctypes.c_double(), ctypes.c_double()
     _lib.check("probe", _lib.load().emsa_probe_rates(ctypes.byref(pe), ctypes.byref(pf)))
+    c_exp, c_m4, c_m16 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _lib.check("probe", _lib.load().emsa_probe_issue_cycles(
+        ctypes.byref(c_exp), ctypes.byref(c_m4), ctypes.byref(c_m16)))
     pair_rate = npairs.value / (us.value * 1e-6)
     achieved = pair_rate * FLOPS_PER_PAIR / 1e12
-    # ceilings of the per-pair instruction mix, each step's pairs at its own
-    # VALU count (compensated difference below sigma^2 = 2e-3): VALU issue
-    # alone (the exp co-issues, as the measured kernel shows) and VALU + exp
-    # fully serialised
+    # Issue-model ceiling: SIMD cycles per 64 pairs of each form at the
+    # measured costs (FMA 4, exp, 4x4x1 and 16x16x4 matrix instructions in
+    # mixed code; the VALU and the matrix cores do not overlap, so the costs
+    # add), each step's pairs split by form; the device issues P_fma / 16 SIMD
+    # cycles per second (64 lanes per 4-cycle FMA).
     valu = np.where(per_step[:, 0] < COMP_BELOW_SIGMA2, VALU_PER_PAIR, VALU_PER_PAIR_PLAIN)
-    t_issue = float(np.sum(per_step[:, 2] * valu)) / pf.value
-    t_serial = t_issue + float(np.sum(per_step[:, 2])) / pe.value
-    ceiling = seq_pairs.value / t_issue
-    ceiling_serial = seq_pairs.value / t_serial
+    cyc_diff = 4.0 * valu + c_exp.value + c_m4.value
+    cyc_exp = 4.0 * VALU_PER_PAIR_EXPANDED + c_exp.value + c_m4.value + 0.25 * c_m16.value
+    p_exp = per_step[:, 3]
+    p_dif = per_step[:, 2] - p_exp
+    t_model = float(np.sum(p_dif * cyc_diff + p_exp * cyc_exp)) / 64.0 / (pf.value / 16.0)
+    ceiling = seq_pairs.value / t_model
 
     out = {
         "metric": METRIC,
@@ -638,8 +648,10 @@ def main():
                                    for k in range(B)),
         "roofline": {
             "bound": "mfma",
-            "compute": "FP32 VALU (difference, |d|^2, C) + v_exp_f32; B and W as a product "
-                       "over the column dimension on the matrix cores (v_mfma_f32_4x4x1f32)",
+            "compute": "FP32 VALU (difference, |d|^2, C) + v_exp_f32, or the exponent expanded "
+                       "about the wave's row centre on v_mfma_f32_16x16x4f32; B and W as a "
+                       "product over the column dimension on the matrix cores "
+                       "(v_mfma_f32_4x4x1f32)",
             "kernel": "emicp_affinity_mfma_kernel",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
@@ -655,11 +667,13 @@ def main():
             "algorithmic": "%d FP32 flops + 1 exp per (x, y) pair" % FLOPS_PER_PAIR,
             "exp_rate_per_s": pe.value,
             "fma_rate_per_s": pf.value,
-            "valu_per_pair": float(np.sum(per_step[:, 2] * valu) / np.sum(per_step[:, 2])),
-            "matrix_instructions_per_pair": 1.0 / 64,
+            "issue_cycles": {"fma": 4.0, "exp": c_exp.value, "mfma_4x4x1f32": c_m4.value,
+                             "mfma_16x16x4f32": c_m16.value},
+            "frac_pairs_expanded": float(np.sum(p_exp) / np.sum(per_step[:, 2])),
+            "simd_cycles_per_64_pairs": float(np.sum(p_dif * cyc_diff + p_exp * cyc_exp) /
+                                              np.sum(per_step[:, 2])),
             "pair_rate_per_s": pair_rate,
             "pair_ceiling_per_s": ceiling,
-            "pair_ceiling_serial_per_s": ceiling_serial,
             "frac_of_binding_ceiling": pair_rate / ceiling,
             # the unfused count: 11 VALU instructions per pair (3 sub, mul + 2 FMA
             # for |d|^2, scale, C, B and 3 W accumulations), or the exp rate

@@ -78,6 +78,7 @@ SIGNATURES = {
     "emsa_emicp_dev_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                          _pp, C.c_void_p]),
     "emsa_probe_rates": (C.c_int, [_dp, _dp]),
+    "emsa_probe_issue_cycles": (C.c_int, [_dp, _dp, _dp]),
     "emsa_emicp_batched": (C.c_int, [C.POINTER(emsa_pair), C.c_int, _fp, C.c_int, _pp, _fp,
                                      _dp]),
     "emsa_emicp_batched_dev": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_void_p, C.c_int,

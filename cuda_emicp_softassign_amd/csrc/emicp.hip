@@ -461,8 +461,9 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // columns, lane jj tests group jj's box against the box of the wave's valid
 // rows, and the loop visits only the groups with a pair whose exp2 argument
 // can be above -kCullArg (a ballot mask, walked lowest bit first); a piece
-// with no such group is not even staged. visits (nullable, measurement): receives the
-// number of (valid row, valid column) pairs the launch evaluates.
+// with no such group is not even staged. visits (nullable, measurement): visits[0]
+// receives the number of (valid row, valid column) pairs the launch evaluates,
+// visits[1] those evaluated in the expanded form.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 // row stride of a wave's stage: 4 floats of padding put the four component
 // rows, which lanes 4b + j read together (row j), in different LDS banks
@@ -770,8 +771,12 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     }
 #pragma unroll
     for (int a = 1; a < NACC; ++a) D[0] += D[a];
-    if (visits && lane == 0 && nvis > 0)
-        atomicAdd(visits, (unsigned long long)nvis * (unsigned long long)min(kEmRows, M - rb * kEmRows));
+    if (visits && lane == 0 && nvis > 0) {
+        const unsigned long long np =
+            (unsigned long long)nvis * (unsigned long long)min(kEmRows, M - rb * kEmRows);
+        atomicAdd(visits, np);
+        if (expd) atomicAdd(visits + 1, np);
+    }
 
     // Combine the 4 waves in LDS, fixed order; wave 0 stores. Lane 4b + j
     // holds component j of rows 4b .. 4b + 3: j = 0, 1, 2 -> W_x, W_y, W_z
@@ -1086,7 +1091,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it], s));
         const EmRowFuse fz{outlier, last, (int)it};
         launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig, s,
-                        (probe && probe->visits) ? probe->visits + it : nullptr,
+                        (probe && probe->visits) ? probe->visits + 2 * it : nullptr,
                         fuse ? &fz : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));

@@ -238,8 +238,8 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
 // dRt0: [nregs][12] initial (R, t); dRtOut: [nregs][12]; dResid: [nregs] or null.
 // Measurement hooks of one enqueued sequence (eager launches only): events
 // (nullable, 2 per sigma^2 step) recorded around each affinity launch, and
-// device counters (nullable, one per sigma^2 step) that the affinity launches
-// add their evaluated pairs to.
+// device counters (nullable, two per sigma^2 step) that the affinity launches
+// add their evaluated pairs to (all, and those of the expanded exponent).
 struct EmStepProbe {
     hipEvent_t* events;
     unsigned long long* visits;
@@ -250,8 +250,9 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
 
 // Single launches of the setup and affinity kernels (measurement probes).
 int em_launch_setup(EmWork& w, const float* dRt0, hipStream_t s);
-// visits (nullable): receives the number of pairs the launch evaluates (the
-// matrix-core kernel; pairs in skipped column groups are not counted).
+// visits (nullable, two counters): receive the number of pairs the launch
+// evaluates (the matrix-core kernel; pairs in skipped column groups are not
+// counted) and those evaluated in the expanded form.
 int em_launch_affinity(const DeviceCtx& ctx, EmWork& w, float sigma_p2, hipStream_t s,
                        unsigned long long* visits = nullptr);
 int default_tile_cols(const DeviceCtx& ctx);

@@ -15,6 +15,14 @@ extern "C" {
  * lane per second, i.e. 2 flops each). Returns EMSA_OK or a negative code. */
 int emsa_probe_rates(double *exp_per_s, double *fma_per_s);
 
+/* Issue cost, in SIMD cycles per wave instruction (an FP32 FMA of a wave of
+ * 64 lanes taking 4), of v_exp_f32, v_mfma_f32_4x4x1f32 and
+ * v_mfma_f32_16x16x4f32 on the current device, measured in loops where each
+ * instruction is mixed with independent FMAs (as in the fused kernels: the
+ * matrix cores and the VALU do not run at the same time, so their costs add).
+ * Returns EMSA_OK or a negative code. */
+int emsa_probe_issue_cycles(double *exp_cycles, double *mfma4_cycles, double *mfma16_cycles);
+
 /* Average duration in microseconds of one launch of the fused EM-ICP
  * affinity kernel over the batch `pairs` (device SoA pointers) x n_starts
  * identity starts, laid out exactly as emsa_emicp_batched_dev lays it out
@@ -45,7 +53,8 @@ int emsa_probe_set_affinity_path(int use_matrix_cores);
  * evaluates, once with HIP events around each affinity launch. Receives the
  * summed affinity time in microseconds, the summed evaluated pairs and the
  * number of affinity launches (sigma^2 steps); per_step (nullable) receives,
- * for the first per_step_cap steps, (sigma^2, microseconds, pairs) each. */
+ * for the first per_step_cap steps, (sigma^2, microseconds, pairs, pairs of
+ * waves that took the expanded exponent) each. */
 int emsa_probe_emicp_sequence(const struct emsa_pair *pairs, int count, int n_starts,
                               const struct emsa_params *params, double *affinity_us,
                               double *pairs_evaluated, int *launches, double *per_step,
