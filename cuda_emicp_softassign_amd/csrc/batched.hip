@@ -29,7 +29,14 @@ void push_params(std::vector<uint64_t>& k, const emsa_params& p) {
     k.push_back(fbits(p.d_02));
 }
 
+// Waits for the entry's last replay (on whatever stream it ran), then frees
+// it; other work on the device is not waited for.
 void drop_entry(GraphEntry& e) {
+    if (e.done) {
+        (void)hipEventSynchronize(e.done);
+        (void)hipEventDestroy(e.done);
+        e.done = nullptr;
+    }
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     if (e.graph) (void)hipGraphDestroy(e.graph);
     e.exec = nullptr;
@@ -56,7 +63,11 @@ int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pa
     for (auto& e : ctx.graphs) {
         if (e->key == key) {
             e->lastUse = g_clock;
+            // the entry's workspace is shared by its replays: a replay on
+            // another stream starts after the previous one (a device-side wait)
+            EMSA_HIP_TRY(hipStreamWaitEvent(user, e->done, 0));
             EMSA_HIP_TRY(hipGraphLaunch(e->exec, user));
+            EMSA_HIP_TRY(hipEventRecord(e->done, user));
             return EMSA_OK;
         }
     }
@@ -86,7 +97,6 @@ int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pa
     }
     ctx.seen.erase(seen);
     if (ctx.graphs.size() >= kMaxGraphs) {  // evict the least recently used
-        EMSA_HIP_TRY(hipDeviceSynchronize());
         auto it = std::min_element(ctx.graphs.begin(), ctx.graphs.end(),
                                    [](const std::unique_ptr<GraphEntry>& a,
                                       const std::unique_ptr<GraphEntry>& b) {
@@ -97,6 +107,7 @@ int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pa
     }
     auto e = std::make_unique<GraphEntry>();
     e->key = key;
+    EMSA_HIP_TRY(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
     int rc = em_prepare(ctx, e->work, pairs, count, n_starts, tileCols, nullptr, nullptr,
                         ctx.stream);
     if (rc != EMSA_OK) {
@@ -119,6 +130,7 @@ int run_graph(DeviceCtx& ctx, const std::vector<uint64_t>& key, const EmPair* pa
     }
     e->lastUse = g_clock;
     EMSA_HIP_TRY(hipGraphLaunch(e->exec, user));
+    EMSA_HIP_TRY(hipEventRecord(e->done, user));
     ctx.graphs.push_back(std::move(e));
     return EMSA_OK;
 }
@@ -356,8 +368,7 @@ int emsa_graph_cache_clear(void) {
     DeviceCtx* ctx = current_ctx();
     if (!ctx) return EMSA_ENODEV;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    EMSA_HIP_TRY(hipDeviceSynchronize());
-    for (auto& e : ctx->graphs) drop_entry(*e);
+    for (auto& e : ctx->graphs) drop_entry(*e);  // each waits for its own last replay
     ctx->graphs.clear();
     ctx->seen.clear();
     return EMSA_OK;

@@ -147,6 +147,7 @@ struct GraphEntry {
     EmWork work;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each replay: the entry's last use
     uint64_t lastUse = 0;
 };
 

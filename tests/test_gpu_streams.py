@@ -80,3 +80,45 @@ def test_softassign_unrolled_two_streams(UR):
     probs = [_sa_problem(400, 380, 320, 2), _sa_problem(600, 520, 330, 1)]
     print("softassign unrolled on two streams: B=2 400x380, B=1 600x520", flush=True)
     _two_streams(lambda X, Y: UR.softassign_unrolled(X, Y, JMAX=6, I0=1, I1=3), probs)
+
+
+def test_cached_sequence_replays_on_two_streams():
+    """A recorded batched sequence (same buffers, sizes and parameters) owns
+    one workspace: replayed on two streams back to back with no
+    synchronisation in between, the second replay starts on the device after
+    the first, and the outputs equal the eager run's bit for bit."""
+    import cuda_emicp_softassign_amd as E
+    pairs = []
+    for k, (n, m) in enumerate([(900, 700), (1500, 1300), (400, 600)]):
+        X, Y, _, _ = range_scans(n, m, seed=300 + k)
+        pairs.append((X, Y))
+    Rt0 = np.stack([E.multistart_rt0(X, Y, [np.eye(3)]) for X, Y in pairs]).astype(np.float32)
+    p = E.registrationParameters(sigma_inf=2e-4)
+    ref, ref_res = E.emicp_batched(pairs, Rt0, p)
+    keep, dpairs = [], []
+    for X, Y in pairs:
+        dx = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+        dy = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()
+        keep += [dx, dy]
+        dpairs.append((dx.data_ptr(), len(X), dy.data_ptr(), len(Y)))
+    d0 = torch.from_numpy(Rt0.reshape(-1)).cuda()
+    dout = torch.zeros(12 * len(pairs), device="cuda")
+    dres = torch.zeros(len(pairs), dtype=torch.float64, device="cuda")
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    for _ in range(2):  # eager, then recorded and replayed
+        E.emicp_batched_dev(dpairs, d0.data_ptr(), 1, dout.data_ptr(), dres.data_ptr(), p,
+                            main.cuda_stream)
+        torch.cuda.synchronize()
+    for rep in range(3):
+        dout.fill_(float("nan"))
+        dres.fill_(float("nan"))
+        torch.cuda.synchronize()
+        E.emicp_batched_dev(dpairs, d0.data_ptr(), 1, dout.data_ptr(), dres.data_ptr(), p,
+                            main.cuda_stream)
+        E.emicp_batched_dev(dpairs, d0.data_ptr(), 1, dout.data_ptr(), dres.data_ptr(), p,
+                            side.cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(dout.cpu().numpy().reshape(len(pairs), 1, 12), ref), rep
+        assert np.array_equal(dres.cpu().numpy().reshape(len(pairs), 1), ref_res), rep
+    E.graph_cache_clear()
