@@ -169,6 +169,28 @@ def counters_mode(which, E, _lib, torch):
             ctypes.byref(seq_n), None, 0))
         out.update(avg_launch_us=seq_us.value / seq_n.value,
                    pairs_per_launch=seq_pairs.value / seq_n.value, launches=seq_n.value)
+    elif which == "affinity-forms":
+        # the 8 x 10k batch at sigma^2 = 1e-2 (every column group evaluated):
+        # 12 launches with the expanded exponent (every wave qualifies), then
+        # 12 with the difference form (warm-up, counted and 10 timed each)
+        keep = []
+        arr = (_lib.emsa_pair * 8)()
+        for k in range(8):
+            X, Y, _, _ = uniform_cube(10000, seed=10 + k)
+            dx = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+            dy = torch.from_numpy(np.ascontiguousarray(Y.T)).to(dev)
+            keep += [dx, dy]
+            arr[k] = _lib.emsa_pair(dx.data_ptr(), 10000, dy.data_ptr(), 10000)
+        L = _lib.load()
+        try:
+            for tag, bound in (("expanded", 32.0), ("difference", 0.0)):
+                _lib.check("expanded", L.emsa_probe_set_affinity_expanded(bound))
+                us, npairs = ctypes.c_double(), ctypes.c_double()
+                _lib.check("probe_affinity", L.emsa_probe_emicp_affinity(
+                    arr, 8, 1, ctypes.c_float(1e-2), 10, ctypes.byref(us), ctypes.byref(npairs)))
+                out[tag] = {"us_per_launch": us.value, "pairs_per_launch": npairs.value}
+        finally:
+            _lib.check("expanded", L.emsa_probe_set_affinity_expanded(32.0))
     elif which in ("softassign", "softassign-log"):
         X, Y, _, _ = surface_with_outliers(5032, seed=1)
         p = E.registrationParameters()
@@ -485,7 +507,8 @@ def main():
     ap.add_argument("--cull-sweep", action="store_true",
                     help="only the EM-ICP affinity of the 8 x 10k batch at a range of sigma^2, "
                          "with and without skipping zero column groups")
-    ap.add_argument("--counters", choices=["affinity", "softassign", "softassign-log",
+    ap.add_argument("--counters", choices=["affinity", "affinity-forms", "softassign",
+                                           "softassign-log",
                                            "softassign-20k"], default=None,
                     help="counter-collection mode: run only this kernel's workload with direct "
                          "launches (no graph replay), then a 256 MiB device copy that "

@@ -692,7 +692,8 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                 const f4 qz = *reinterpret_cast<const f4*>(&st[2][cq]);
                 f4 nw2;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) nw2[v] = -fmaf(qx[v], qx[v], fmaf(qy[v], qy[v], qz[v] * qz[v]));
+                for (int v = 0; v < 4; ++v)  // -|w|^2, the signs on the FMA operands
+                    nw2[v] = fmaf(-qx[v], qx[v], fmaf(-qy[v], qy[v], -qz[v] * qz[v]));
 #pragma unroll
                 for (int sb = 0; sb < 4; ++sb) {
                     const f4 ex = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[sb], nw2, 0, 0, 0);
