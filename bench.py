@@ -47,9 +47,9 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
 # FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
 # WRITE_SIZE, run on `bench.py --counters affinity`: both instances fetch
-# 15.6 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and write 8.4 MB of
-# partial sums; no register spills
-AFFINITY_TRAFFIC_BYTES = 24.0e6
+# 15.7 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and write 8.4-8.8 MB
+# of partial sums; no register spills
+AFFINITY_TRAFFIC_BYTES = 24.2e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
