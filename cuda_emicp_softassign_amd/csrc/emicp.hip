@@ -79,14 +79,21 @@ __device__ inline int find_reg(const EmRegDesc* __restrict__ regs, int nregs, in
 // (kOrdBits bits per axis), so that consecutive points are close in space: a
 // wave's 64 source rows and each group of kCullCols target columns then have
 // small bounding boxes. The affinity kernel skips a column group for a wave
-// when the boxes are so far apart that exp2(-|d|^2 k) underflows to exactly
-// 0 for every pair of the group (e = 0 adds nothing to C, B or W), so every
-// sum stays bit for bit what the full loop over the same order gives. Per-row
-// results are written back in the caller's row order.
+// when the boxes are so far apart that every pair of the group has
+// e = exp2(-|d|^2 k) <= 2^-64, i.e. an affinity A = e^2 (the reference's
+// expf(-|d|^2 / sigma^2)) below 2^-128, under FLT_MIN: the reference's
+// fast-math build flushes such an A to exactly 0, and an FP32 sum of such
+// terms changes a normaliser that is above the floor (C > 1e-6) by less than
+// 1e-28 relative. A second mode skips only the groups whose e underflow to
+// exactly 0 (2^-150 already rounds to 0), so that every sum stays bit for bit
+// what the full loop over the same order gives. Per-row results are written
+// back in the caller's row order.
 // ---------------------------------------------------------------------------
-// |exp2 argument| beyond which e is 0 in FP32 (2^-150 already rounds to 0);
-// the margin covers the rounding of the box distance.
-constexpr float kCullArg = 160.f;
+// |exp2 argument| of e beyond which a group is skipped: mode 1 (the default)
+// 64 (A < 2^-128 < FLT_MIN), mode 2 160 (e exactly 0 in FP32); the margins
+// cover the rounding of the box distance.
+constexpr float kCullArgUnderflow = 64.f;
+constexpr float kCullArgExact = 160.f;
 
 // One workgroup per cloud: blockIdx.x = 2 * registration + (0: target,
 // 1: source); only the registration that owns a pair's buffers orders it.
@@ -460,8 +467,9 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // cull != 0 (and boxes present): per piece of up to 32 groups of kCullCols
 // columns, lane jj tests group jj's box against the box of the wave's valid
 // rows, and the loop visits only the groups with a pair whose exp2 argument
-// can be above -kCullArg (a ballot mask, walked lowest bit first); a piece
-// with no such group is not even staged. visits (nullable, measurement): visits[0]
+// can be above -kCullArgUnderflow (cull 1) or -kCullArgExact (cull 2) (a
+// ballot mask, walked lowest bit first); a piece with no such group is not
+// even staged. visits (nullable, measurement): visits[0]
 // receives the number of (valid row, valid column) pairs the launch evaluates,
 // visits[1] those evaluated in the expanded form.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
@@ -519,6 +527,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     const bool rv = r < M;
     const float nk = ka * ka;
     bool doCull = cull && d.gbox;
+    const float cullArg = (cull == 2) ? kCullArgExact : kCullArgUnderflow;
     float rlo[3] = {0.f, 0.f, 0.f}, rhi[3] = {0.f, 0.f, 0.f};
     bool finiteRows = false;
     if (doCull || xmax > 0.f) {
@@ -631,7 +640,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                 const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
                 const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
                 const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-                kg = !(g2 * nk > kCullArg);
+                kg = !(g2 * nk > cullArg);
             }
             keep = (unsigned)__ballot(kg);
             if (keep == 0) continue;
@@ -668,7 +677,8 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         wave_lds_sync();
         if constexpr (EXP) {
             // 16-column steps: slots 2m and 2m + 1 together (a slot the box
-            // test rejected has every e exactly 0 in this form too)
+            // test rejected has every e below the threshold in this form too:
+            // with |u|^2 <= xmax its exponent's rounding is far below 1)
             unsigned km = (keep | (keep >> 1)) & 0x55555555u;
             while (km) {  // wave-uniform
                 const int jj = __builtin_ctz(km);

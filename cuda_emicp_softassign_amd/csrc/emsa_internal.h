@@ -185,7 +185,7 @@ struct DeviceCtx {
     bool emPending = false;
     int emTileCols = 0;        // batched column tile (0: heuristic); set by the autotuner
     int emAffinityMfma = 1;    // 1: B and W of the affinity on the matrix cores; 0: on the VALU
-    int emCull = 1;            // 1: skip column groups whose affinities all underflow to 0
+    int emCull = 1;            // skip column groups: 1 all A < 2^-128 (< FLT_MIN), 2 all e == 0, 0 none
     int emRowFuse = 1;         // 1: single-chunk tiles finalize their rows in the affinity kernel
     float emExpanded = 32.f;   // affinity exponent on the matrix cores while ka^2 a^2 <= this (0: never)
     int emBlocksPerCU = 0;     // plain plan occupancy target (0: heuristic)

@@ -239,13 +239,13 @@ extern "C" int emsa_probe_set_affinity_path(int use_matrix_cores) {
     return emsa_graph_cache_clear();  // recorded sequences hold the previous kernel
 }
 
-extern "C" int emsa_probe_set_affinity_cull(int enable) {
+extern "C" int emsa_probe_set_affinity_cull(int mode) {
     clear_error();
     DeviceCtx* ctx = current_ctx();
     if (!ctx) return EMSA_ENODEV;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->emCull = enable ? 1 : 0;
+        ctx->emCull = (mode == 2) ? 2 : (mode ? 1 : 0);
     }
     return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
 }

@@ -67,12 +67,16 @@ int emsa_probe_emicp_sequence(const struct emsa_pair *pairs, int count, int n_st
  * tests of the ordering. */
 int emsa_probe_hilbert_order(const float *dP, int n, int *dPerm);
 
-/* Matrix-core EM-ICP affinity of the current device: enable != 0 (the default)
- * skips, per wave of 64 source rows, the groups of 8 target columns whose
- * every affinity underflows to exactly 0 in FP32 (bounding boxes of the clouds
- * in Hilbert order); 0 visits every column. Both give bit-identical results.
- * For A/B measurement; drops the recorded launch sequences of the device. */
-int emsa_probe_set_affinity_cull(int enable);
+/* Matrix-core EM-ICP affinity of the current device: which groups of 8 target
+ * columns a wave of 64 source rows skips (bounding boxes of the clouds in
+ * Hilbert order). 1 (the default): the groups whose every affinity
+ * A = exp(-|d|^2 / sigma^2) is below 2^-128, under FLT_MIN (the reference's
+ * fast-math build flushes such an A to 0; the sums change by far less than
+ * FP32 resolves); 2: only the groups whose every e = sqrt(A) underflows to
+ * exactly 0 in FP32, which gives results bit-identical to 0; 0 visits every
+ * column. Any other nonzero value is 1. For A/B measurement; drops the
+ * recorded launch sequences of the device. */
+int emsa_probe_set_affinity_cull(int mode);
 
 /* Matrix-core EM-ICP affinity of the current device: a wave of 64 source rows
  * forms its exponents -|x - p|^2 / (2 sigma^2) on the matrix cores, expanded

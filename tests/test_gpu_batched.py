@@ -196,11 +196,7 @@ def test_affinity_paths_agree(E):
     assert geodesic(R1v.astype(float), R1m.astype(float)) < 1e-5
 
 
-def test_affinity_cull_bit_identical(E):
-    """Skipping the target column groups whose affinities all underflow to 0
-    (the default) changes no bit of any result: the per-step traces of a plain
-    registration, and the batched multi-start results and residuals, equal
-    those of the loop over every column in the same (Hilbert) order."""
+def _cull_runs(E, mode):
     from cuda_emicp_softassign_amd import _lib
     L = _lib.load()
     X, Y, _, _ = uniform_cube(3000, seed=21)
@@ -209,16 +205,45 @@ def test_affinity_cull_bit_identical(E):
     Rt0 = np.stack([E.multistart_rt0(a, b, starts) for a, b in pairs])
     p = E.registrationParameters()  # full schedule down to sigma^2 = 1e-5
     try:
-        _lib.check("cull", L.emsa_probe_set_affinity_cull(0))
-        Rf, tf, trf = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p)
-        Rbf, rbf = E.emicp_batched(pairs, Rt0, p)
+        _lib.check("cull", L.emsa_probe_set_affinity_cull(mode))
+        out = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p) + E.emicp_batched(pairs, Rt0, p)
     finally:
         _lib.check("cull", L.emsa_probe_set_affinity_cull(1))
-    Rc, tc, trc = E.emicp_traced(X, Y, np.eye(3), np.zeros(3), p)
-    Rbc, rbc = E.emicp_batched(pairs, Rt0, p)
+    return out
+
+
+def test_affinity_cull_exact_bit_identical(E):
+    """Skipping only the target column groups whose affinities all underflow
+    to exactly 0 (mode 2) changes no bit of any result: the per-step traces
+    of a plain registration, and the batched multi-start results and
+    residuals, equal those of the loop over every column in the same
+    (Hilbert) order."""
+    Rf, tf, trf, Rbf, rbf = _cull_runs(E, 0)
+    Rc, tc, trc, Rbc, rbc = _cull_runs(E, 2)
     assert np.array_equal(trf, trc)
     assert np.array_equal(Rf, Rc) and np.array_equal(tf, tc)
     assert np.array_equal(Rbf, Rbc) and np.array_equal(rbf, rbc)
+
+
+def test_affinity_cull_default_matches_full_loop(E):
+    """The default skipping (mode 1: groups whose every affinity A = e^2 is
+    below 2^-128, under FLT_MIN) against the loop over every column: per-step
+    Lambda within 1e-7 relative and (R, t) within 1e-7 after every step of a
+    plain registration; batched multi-start results within 1e-6 rad and
+    residuals within 1e-6 relative on the well-posed pairs (2.5k x 700,
+    1007 x 1007, 4100 x 3900 and the 3000-point cube)."""
+    Rf, tf, trf, Rbf, rbf = _cull_runs(E, 0)
+    Rc, tc, trc, Rbc, rbc = _cull_runs(E, 1)
+    dl = np.max(np.abs(trc[:, 1] - trf[:, 1]) / trf[:, 1])
+    dR = np.max(np.abs(trc[:, 17:29] - trf[:, 17:29]))
+    print("default skipping vs every column: Lambda %.1e, (R, t) %.1e" % (dl, dR), flush=True)
+    assert dl < 1e-7 and dR < 1e-7
+    for k in (1, 2, 4, 5):
+        for st in range(2):
+            err = geodesic(Rbc[k, st, :9].reshape(3, 3).astype(float),
+                           Rbf[k, st, :9].reshape(3, 3).astype(float))
+            assert err < 1e-6, (k, st, err)
+            assert abs(rbc[k, st] - rbf[k, st]) <= 1e-6 * abs(rbf[k, st]), (k, st)
 
 
 def test_affinity_cull_non_finite_inputs(E):

@@ -279,6 +279,27 @@ def test_large_matches_oracle_first_steps(E, oracle):
     assert not tr[:, 31].any()  # no failed Procrustes step
 
 
+def test_large_matches_oracle_full_schedule(E, oracle):
+    """N = M = 10000 (the benchmark's pair 0), the full 66-step schedule
+    against the FP64 oracle run from the same start: R within 1e-4 rad and t
+    within 1e-5 of the extent after every step, the final registration within
+    1e-6 rad, and Lambda of every step within 1e-5 relative."""
+    X, Y, Rt, tt = uniform_cube(10000, seed=10)
+    Ro, to, tro = oracle.emicp(X.astype(np.float64), Y.astype(np.float64))
+    R, t, tr = E.emicp_traced(X, Y, np.eye(3), np.zeros(3))
+    assert len(tr) == len(tro) == 66
+    extent = float(np.ptp(X, axis=0).max())
+    rot = [geodesic(tr[k, 17:26].reshape(3, 3), tro[k, 17:26].reshape(3, 3)) for k in range(66)]
+    dt = [np.abs(tr[k, 26:29] - tro[k, 26:29]).max() / extent for k in range(66)]
+    dl = [abs(tr[k, 1] - tro[k, 1]) / tro[k, 1] for k in range(66)]
+    print("10k full schedule: max per-step rot err %.2e rad, |dt|/extent %.2e, Lambda %.2e; "
+          "final %.2e rad" % (max(rot), max(dt), max(dl), geodesic(R.astype(np.float64), Ro)),
+          flush=True)
+    assert max(rot) < 1e-4 and max(dt) < 1e-5 and max(dl) < 1e-5
+    assert geodesic(R.astype(np.float64), Ro) < 1e-6
+    assert np.abs(t - to).max() < 1e-5 * extent
+
+
 def test_dropin_cxx_caller():
     """tests/cpp/dropin_main.cpp, a caller of the reference API compiled
     against include/3dregistration.h and linked to libemsa.so."""
