@@ -626,23 +626,37 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     // the column loop, once per form of the exponent (a wave-uniform choice)
     auto columns = [&](auto expTag) {
     constexpr bool EXP = decltype(expTag)::value;
+    // skipping masks of the current batch of 4 pieces (wave group k tested
+    // by lane k - kb and k - kb - 64: bit k - kb of cm0, of cm1 past 64)
+    unsigned long long cm0 = 0, cm1 = 0;
+    auto group_live = [&](int k) {  // group G0 + wave + 4k has a pair above the threshold
+        const float* bx = d.gbox + (size_t)(G0 + wave + kEmWaves * k) * 8;
+        const float4 lo = *reinterpret_cast<const float4*>(bx);
+        const float4 hi = *reinterpret_cast<const float4*>(bx + 4);
+        const float gx = fmaxf(fmaxf(rlo[0] - hi.x, lo.x - rhi[0]), 0.f);
+        const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
+        const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
+        const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+        return !(g2 * nk > cullArg);
+    };
     for (int k0 = 0; k0 < nw; k0 += kPieceGroups) {
         const int ng = min(kPieceGroups, nw - k0);
         const int gbase = G0 + wave + kEmWaves * k0;  // group of piece slot jj: gbase + 4 jj
         unsigned keep = (ng >= 32) ? 0xFFFFFFFFu : ((1u << ng) - 1u);
         if (doCull) {
-            bool kg = false;
-            if (lane < ng) {
-                const float* bx = d.gbox + (size_t)(gbase + kEmWaves * lane) * 8;
-                const float4 lo = *reinterpret_cast<const float4*>(bx);
-                const float4 hi = *reinterpret_cast<const float4*>(bx + 4);
-                const float gx = fmaxf(fmaxf(rlo[0] - hi.x, lo.x - rhi[0]), 0.f);
-                const float gy = fmaxf(fmaxf(rlo[1] - hi.y, lo.y - rhi[1]), 0.f);
-                const float gz = fmaxf(fmaxf(rlo[2] - hi.z, lo.z - rhi[2]), 0.f);
-                const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-                kg = !(g2 * nk > cullArg);
+            // the first piece of a batch tests the batch's 128 groups at once
+            // (two per lane, their box loads in flight together): one memory
+            // round trip per 4 pieces instead of one per piece
+            const int kb = k0 & ~(4 * kPieceGroups - 1);
+            if (k0 == kb) {
+                const int ka0 = kb + lane, ka1 = kb + kWave + lane;
+                const bool l0 = (ka0 < nw) && group_live(ka0);
+                const bool l1 = (ka1 < nw) && group_live(ka1);
+                cm0 = __ballot(l0);
+                cm1 = __ballot(l1);
             }
-            keep = (unsigned)__ballot(kg);
+            const int pi = (k0 - kb) / kPieceGroups;  // 0..3
+            keep &= (unsigned)(((pi < 2) ? cm0 : cm1) >> (32 * (pi & 1)));
             if (keep == 0) continue;
         }
         // stage [x; y; z; 1] of the piece (EXP: [w; 1], and slots past the
@@ -888,6 +902,10 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
         const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
         const double outlier = fuse ? fuse->outlier : 0.0;
         const int last = fuse ? fuse->last : 0, iter = fuse ? fuse->iter : 0;
+// (the instances that finalize rows in their epilogue hold one register pair
+// in scratch between the row setup and the epilogue: one store and one load
+// per lane; 2 accumulators would avoid it at the same launch times, but the
+// sums must be those of the row kernel's path bit for bit)
 #define EMSA_AFF(C_, F_)                                                                     \
     hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 4, F_>), dim3(tiles), blk, 0, s, desc, \
                        nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter)
