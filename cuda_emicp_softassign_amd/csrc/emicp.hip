@@ -460,8 +460,10 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // without) + 1 exp of -|d'|^2 (the sign is a source modifier), and
 // one matrix instruction per column and wave. UNR columns per loop iteration,
 // NACC independent accumulators (measured on MI355X, 8 x 10k batched
-// sequence: NACC 4 is 4% faster than 2 and 1, 8 is 8% slower; UNR is the
-// column group of the skipping, 8). One matrix instruction per column and
+// sequence: with the difference form only and 2048-column tiles NACC 4 was 4%
+// faster than 2 and 1, 8 was 8% slower; with the expanded form, which has its
+// own accumulators, 2 and 4 run the same; UNR is the column group of the
+// skipping, 8). One matrix instruction per column and
 // wave costs about 16 SIMD cycles here: moving C = sum e^2 onto
 // the matrix cores as well (the diagonal of e e^T) made the launch 30% slower.
 // cull != 0 (and boxes present): per piece of up to 32 groups of kCullCols
@@ -902,12 +904,12 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
         const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
         const double outlier = fuse ? fuse->outlier : 0.0;
         const int last = fuse ? fuse->last : 0, iter = fuse ? fuse->iter : 0;
-// (the instances that finalize rows in their epilogue hold one register pair
-// in scratch between the row setup and the epilogue: one store and one load
-// per lane; 2 accumulators would avoid it at the same launch times, but the
-// sums must be those of the row kernel's path bit for bit)
+// 2 matrix accumulators: with 4, the instances that finalize rows in their
+// epilogue held one register pair in scratch between the row setup and the
+// epilogue (2.6 MB written per batched 8 x 10k launch); 2 give the same launch
+// times for the batched (whole-row and 2048-column tiles) and plain runs
 #define EMSA_AFF(C_, F_)                                                                     \
-    hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 4, F_>), dim3(tiles), blk, 0, s, desc, \
+    hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 2, F_>), dim3(tiles), blk, 0, s, desc, \
                        nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter)
         if (fuse) {
             if (comp) EMSA_AFF(true, true);
