@@ -44,12 +44,12 @@ COMP_BELOW_SIGMA2 = 2e-3
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes per launch of the batched affinity kernel on the default workload
-# (8 x 10k, averaged over the 66 launches of its sequence): rocprofv3 --pmc
-# FETCH_SIZE (x2, from a 256 MiB copy calibration in the same process) +
-# WRITE_SIZE, run on `bench.py --counters affinity`: both instances fetch
-# 15.7 MB (each of the 8 L2s reads the ~1.9 MB of inputs) and write 8.4-8.8 MB
-# of partial sums; no register spills
-AFFINITY_TRAFFIC_BYTES = 24.2e6
+# (8 x 10k in the autotuned whole-row decomposition, averaged over the 66
+# launches of its sequence): rocprofv3 --pmc FETCH_SIZE (x2, from a 256 MiB
+# copy calibration in the same process) + WRITE_SIZE, run on
+# `bench.py --counters affinity`: 11.9 MB fetched (the L2s read the ~1.9 MB of
+# inputs) and 0.8 MB of row-block moments written; no register spills
+AFFINITY_TRAFFIC_BYTES = 12.7e6
 
 
 def small_batched(E, torch, dev, sptr, timed, steps, p, n_pairs=64, n=1007):
@@ -117,7 +117,7 @@ def cull_sweep(_lib, torch):
     from cuda_emicp_softassign_amd.clouds import uniform_cube
     L = _lib.load()
     dev = torch.device("cuda:0")
-    keep = []
+    keep, dpairs = [], []
     arr = (_lib.emsa_pair * 8)()
     for k in range(8):
         X, Y, _, _ = uniform_cube(10000, seed=10 + k)
@@ -125,6 +125,12 @@ def cull_sweep(_lib, torch):
         dy = torch.from_numpy(np.ascontiguousarray(Y.T)).to(dev)
         keep += [dx, dy]
         arr[k] = _lib.emsa_pair(dx.data_ptr(), 10000, dy.data_ptr(), 10000)
+        dpairs.append((dx.data_ptr(), 10000, dy.data_ptr(), 10000))
+    import cuda_emicp_softassign_amd as E
+    # the decomposition the benchmark's headline uses (its autotuner)
+    tile = E.autotune_emicp(dpairs, 1, E.registrationParameters(),
+                            torch.cuda.current_stream().cuda_stream)
+    print(json.dumps({"tile_cols": tile}), flush=True)
     try:
         for s2 in (1e-2, 2e-3, 1e-3, 5e-4, 2e-4, 1e-4, 5e-5, 2e-5, 1.0611e-5):
             row = {"sigma_p2": s2}
