@@ -622,6 +622,16 @@ def main():
         exact_identical = bool(np.array_equal(dout.cpu().numpy().reshape(B, 12), Rfull))
     finally:
         _lib.check("cull", L.emsa_probe_set_affinity_cull(1))
+    # and with the affinity workgroups in tile order instead of the order
+    # sorted by the previous step's work (bit-identical by construction)
+    _lib.check("balance", L.emsa_probe_set_affinity_balance(0))
+    try:
+        for _ in range(warmup):
+            batched()
+        ms_tile_order, _ = timed(batched, args.steps)
+        tile_order_identical = bool(np.array_equal(dout.cpu().numpy().reshape(B, 12), Rb))
+    finally:
+        _lib.check("balance", L.emsa_probe_set_affinity_balance(1))
     full_rot_diff = max(geodesic(Rb[k, :9].reshape(3, 3).astype(float),
                                  Rfull[k, :9].reshape(3, 3).astype(float)) for k in range(B))
 
@@ -687,6 +697,8 @@ def main():
         "t_diff_vs_every_column": float(np.abs(Rb[:, 9:] - Rfull[:, 9:]).max()),
         "batched_registrations_per_s_exact_zero_skipping": B * 1000.0 / ms_exact,
         "exact_zero_skipping_bit_identical": exact_identical,
+        "batched_registrations_per_s_tile_order": B * 1000.0 / ms_tile_order,
+        "tile_order_bit_identical": tile_order_identical,
         "ms_per_registration_plain": ms_plain / B,
         "iterations_per_s": B * n_sigma * 1000.0 / ms_batched,
         "wall_s_timed": wall,

@@ -92,6 +92,7 @@ SIGNATURES = {
     "emsa_probe_set_affinity_path": (C.c_int, [C.c_int]),
     "emsa_probe_set_affinity_cull": (C.c_int, [C.c_int]),
     "emsa_probe_set_row_fuse": (C.c_int, [C.c_int]),
+    "emsa_probe_set_affinity_balance": (C.c_int, [C.c_int]),
     "emsa_probe_set_affinity_expanded": (C.c_int, [C.c_float]),
     "emsa_probe_hilbert_order": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "emsa_probe_emicp_sequence": (C.c_int, [C.POINTER(emsa_pair), C.c_int, C.c_int,

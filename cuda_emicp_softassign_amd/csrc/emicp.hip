@@ -473,7 +473,11 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // ballot mask, walked lowest bit first); a piece with no such group is not
 // even staged. visits (nullable, measurement): visits[0]
 // receives the number of (valid row, valid column) pairs the launch evaluates,
-// visits[1] those evaluated in the expanded form.
+// visits[1] those evaluated in the expanded form. perm (nullable): the tile of
+// each workgroup (else tile = workgroup); work (nullable): receives each
+// tile's visited columns, 3 per column of a wave in the expanded form and 4
+// in the difference form (about their relative issue costs), the key the
+// next step's order is sorted by.
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 // row stride of a wave's stage: 4 floats of padding put the four component
 // rows, which lanes 4b + j read together (row j), in different LDS banks
@@ -486,14 +490,16 @@ template <bool COMP, int UNR, int NACC, bool FUSE>
 __global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
 emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
                            float xmax, unsigned long long* __restrict__ visits, double outlier,
-                           int last, int iter) {
+                           int last, int iter, const int* __restrict__ perm,
+                           int* __restrict__ work) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const int ri = find_reg(regs, nregs, blockIdx.x);
+    const int tile = perm ? perm[blockIdx.x] : (int)blockIdx.x;
+    const int ri = find_reg(regs, nregs, tile);
     const EmRegDesc& d = regs[ri];
     const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
-    const int local = blockIdx.x - d.tileBase;
+    const int local = tile - d.tileBase;
     const int rb = local / chunks;
     const int g = local - rb * chunks;
     const int lane = threadIdx.x & (kWave - 1);
@@ -700,7 +706,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                 const int jj = __builtin_ctz(km);
                 km &= km - 1u;
                 const int c = jj * kCullCols;  // in the stage
-                if (visits) {
+                if (visits || work) {
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int cc0 = (gbase + kEmWaves * (jj + t)) * kCullCols;
@@ -739,7 +745,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
             keep &= keep - 1u;
             const int c = jj * UNR;                               // in the stage
             const int cc0 = (gbase + kEmWaves * jj) * kCullCols;  // in the target
-            if (visits) nvis += max(0, min(UNR, d.N - cc0));
+            if (visits || work) nvis += max(0, min(UNR, d.N - cc0));
             f4 bq[UNR / 4];
 #pragma unroll
             for (int v = 0; v < UNR / 4; ++v)
@@ -811,6 +817,8 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     // its own stage, which it no longer reads (its LDS accesses are in order),
     // so the workgroup needs 16 KB of LDS and 8 waves per SIMD fit.
     static_assert(5 * kEmRows <= 4 * kMfStageLd, "partials fit a wave's stage");
+    __shared__ int waveWork[kEmWaves];
+    if (work && lane == 0) waveWork[wave] = nvis * (expd ? 3 : 4);
     float* red = &stage[wave][0][0];
     red[lane] = C;
     const int q = (j == 3) ? 1 : 2 + j;
@@ -827,6 +835,9 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     }
     __syncthreads();
     if (wave == 0) {
+        static_assert(kEmWaves == 4, "four wave weights");
+        if (work && lane == 0)
+            work[tile] = (waveWork[0] + waveWork[1]) + (waveWork[2] + waveWork[3]);
         float acc[5];
 #pragma unroll
         for (int qq = 0; qq < 5; ++qq) {
@@ -854,9 +865,121 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 }
 
 
-// Stage 3: one workgroup per registration.
+// The next step's affinity workgroup order (one workgroup of kEmRows *
+// kEmWaves threads): the n <= kBalanceMaxTiles tiles in order of their work,
+// heaviest first, in 64 levels (a counting sort: per level and per group of
+// 64 tiles, the count and each tile's rank among the lanes of its level, then
+// prefix sums; equal levels keep the tile order), dealt to the workgroup
+// slots in rounds of `width` (the compute units): even rounds ascending, odd
+// full rounds descending, so that the slots b, b + width, ... that share a
+// compute unit get heavy and light tiles in turn. perm[b] = the tile of
+// workgroup b, a permutation of 0..n-1 for any work values.
+__device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ perm, int n,
+                              int width) {
+    static_assert(kEmRows * kEmWaves == kWave * kEmWaves, "the solve kernel's workgroup");
+    constexpr int kLevels = 64;
+    constexpr int kThreads = kWave * kEmWaves;
+    constexpr int kRounds = kBalanceMaxTiles / kThreads;  // tiles per thread at most
+    constexpr int kRW = kRounds * kEmWaves;               // (round, wave) slots of 64 tiles
+    constexpr int kSeg = kRW / kEmWaves;                  // slots per scanning wave
+    static_assert(kLevels == kWave, "one level per lane in the scans");
+    __shared__ int cnt[kRW][kLevels];
+    __shared__ int segTot[kEmWaves][kLevels];
+    __shared__ int lvBase[kLevels];
+    __shared__ int wmax[kEmWaves];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    // every load in flight at once: tile r kThreads + tid of each round r
+    int wv[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = r * kThreads + tid;
+        wv[r] = (i < n) ? max(work[i], 0) : 0;
+    }
+    for (int e = tid; e < kRW * kLevels; e += kThreads) (&cnt[0][0])[e] = 0;
+    int m = 0;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) m = max(m, wv[r]);
+    for (int o = 1; o < kWave; o <<= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    // level of work w: 63 - floor(w * 64 / (max + 1)), in FP32 (an ordering
+    // heuristic: any rounding still gives levels 0..63)
+    const float top = (float)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])) + 1.f;
+    const float scale = (float)kLevels / top;
+    const int rounds = (n + kThreads - 1) / kThreads;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // per (round, wave): tiles of each level (0: heaviest), and each tile's
+    // rank among the lanes of its level below it (equal levels found by one
+    // ballot per level bit)
+    int lv[kRounds], rk[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        lv[r] = 0;
+        rk[r] = 0;
+        if (r >= rounds) continue;  // uniform
+        const bool valid = r * kThreads + tid < n;
+        const int d = kLevels - 1 - min(kLevels - 1, (int)((float)wv[r] * scale));
+        unsigned long long same = __ballot(valid);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const unsigned long long b = __ballot(valid && ((d >> k) & 1));
+            same &= ((d >> k) & 1) ? b : ~b;
+        }
+        lv[r] = d;
+        rk[r] = __popcll(same & below);
+        if (valid && rk[r] == 0) cnt[r * kEmWaves + wave][d] = __popcll(same);
+    }
+    __syncthreads();
+    {  // exclusive prefix over the slots per level: wave s scans slots [s kSeg, (s + 1) kSeg)
+        int c[kSeg];
+#pragma unroll
+        for (int k = 0; k < kSeg; ++k) c[k] = cnt[wave * kSeg + k][lane];
+        int run = 0;
+#pragma unroll
+        for (int k = 0; k < kSeg; ++k) {
+            cnt[wave * kSeg + k][lane] = run;
+            run += c[k];
+        }
+        segTot[wave][lane] = run;
+    }
+    __syncthreads();
+    if (wave == 0) {  // level totals, then their exclusive prefix over the levels
+        int t = 0;
+        for (int s = 0; s < kEmWaves; ++s) {
+            const int c = segTot[s][lane];
+            segTot[s][lane] = t;
+            t += c;
+        }
+        int inc = t;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += v;
+        }
+        lvBase[lane] = inc - t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = r * kThreads + tid;
+        if (i < n) {
+            const int rw = r * kEmWaves + wave, d = lv[r];
+            const int pos = lvBase[d] + segTot[rw / kSeg][d] + cnt[rw][d] + rk[r];
+            const int round = pos / width, p = pos - round * width;
+            const bool full = (round + 1) * width <= n;
+            perm[round * width + (((round & 1) && full) ? width - 1 - p : p)] = i;
+        }
+    }
+}
+
+// Stage 3: one workgroup per registration; with work != nullptr one more
+// workgroup (the last) orders the next step's affinity workgroups.
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
+emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter, int nregs,
+                   const int* __restrict__ work, int* __restrict__ perm, int ntiles, int width) {
+    if ((int)blockIdx.x >= nregs) {
+        balance_tiles(work, perm, ntiles, width);
+        return;
+    }
     const EmRegDesc& d = regs[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -896,7 +1019,7 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter)
 // the all-VALU kernel.
 void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
                      float sigma_p2, hipStream_t s, unsigned long long* visits,
-                     const EmRowFuse* fuse) {
+                     const EmRowFuse* fuse, const int* perm = nullptr, int* work = nullptr) {
     const dim3 blk(kEmRows * kEmWaves);
     const bool comp = sigma_p2 < kCompBelowSigma2;
     if (ctx.emAffinityMfma) {
@@ -910,7 +1033,8 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
 // times for the batched (whole-row and 2048-column tiles) and plain runs
 #define EMSA_AFF(C_, F_)                                                                     \
     hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 2, F_>), dim3(tiles), blk, 0, s, desc, \
-                       nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter)
+                       nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter, perm, \
+                       work)
         if (fuse) {
             if (comp) EMSA_AFF(true, true);
             else EMSA_AFF(false, true);
@@ -1094,6 +1218,12 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     if ((rc = w.state.reserve(sizeof(double) * kStateW * nregs)) != EMSA_OK) return rc;
     if ((rc = w.desc.reserve(sizeof(EmRegDesc) * nregs)) != EMSA_OK) return rc;
     if (sz.ordW > 0 && (rc = w.order.reserve(4 * sz.ordW)) != EMSA_OK) return rc;
+    double launchPairs = 0.0;
+    for (const EmRegDesc& d : w.regs) launchPairs += (double)d.N * (double)d.M;
+    w.balance = ctx.emBalance && ctx.emAffinityMfma && ctx.emCull && !w.ordOff.empty() &&
+                w.tiles <= kBalanceMaxTiles && launchPairs >= kBalanceMinPairs;
+    if (w.balance && (rc = w.sched.reserve(2 * sizeof(int) * (size_t)w.tiles)) != EMSA_OK)
+        return rc;
     em_bind(w, w.xpad.as<float>(), w.partial.as<float>(), w.moments.as<double>(),
             w.state.as<double>(), sz.ordW > 0 ? w.order.as<unsigned>() : nullptr);
     EMSA_HIP_TRY(hipMemcpyAsync(w.desc.ptr, w.regs.data(), sizeof(EmRegDesc) * nregs,
@@ -1115,6 +1245,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     // others and is not launched when there are none
     const bool fuse = ctx.emRowFuse && ctx.emAffinityMfma && w.someSingleChunk;
     const bool rowsKernel = !(fuse && w.singleChunk);
+    // balanced workgroup order: step 0 in tile order, each later step in the
+    // order the previous step's solve launch sorted from its work
+    int* const work = w.balance ? w.sched.as<int>() : nullptr;
+    int* const perm = w.balance ? w.sched.as<int>() + w.tiles : nullptr;
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
@@ -1123,7 +1257,7 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
         const EmRowFuse fz{outlier, last, (int)it};
         launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig, s,
                         (probe && probe->visits) ? probe->visits + 2 * it : nullptr,
-                        fuse ? &fz : nullptr);
+                        fuse ? &fz : nullptr, it > 0 ? perm : nullptr, work);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
         if (rowsKernel) {
@@ -1131,8 +1265,10 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
                                desc, nregs, w.rowBlocks, outlier, last, (int)it, fuse ? 1 : 0);
             EMSA_HIP_TRY(hipGetLastError());
         }
-        hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,
-                           sig, (int)it);
+        const bool order = work && !last;  // the next step's workgroup order
+        hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs + (order ? 1 : 0)),
+                           dim3(kEmRows * kEmWaves), 0, s, desc, sig, (int)it, nregs, work, perm,
+                           (int)w.tiles, std::max(1, ctx.numCU));
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(emicp_output_kernel, dim3(nregs), dim3(64), 0, s, desc, dRtOut, dResid);

@@ -274,6 +274,17 @@ extern "C" int emsa_probe_set_row_fuse(int enable) {
     return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
 }
 
+extern "C" int emsa_probe_set_affinity_balance(int enable) {
+    clear_error();
+    DeviceCtx* ctx = current_ctx();
+    if (!ctx) return EMSA_ENODEV;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->emBalance = enable ? 1 : 0;
+    }
+    return emsa_graph_cache_clear();  // recorded sequences hold the previous setting
+}
+
 extern "C" int emsa_probe_emicp_sequence(const emsa_pair* pairs, int count, int n_starts,
                                          const emsa_params* params, double* affinity_us,
                                          double* pairs_evaluated, int* launches,

@@ -246,6 +246,44 @@ def test_affinity_cull_default_matches_full_loop(E):
             assert abs(rbc[k, st] - rbf[k, st]) <= 1e-6 * abs(rbf[k, st]), (k, st)
 
 
+def _balance_runs(E, on, tile_cols):
+    from cuda_emicp_softassign_amd import _lib
+    L = _lib.load()
+    pairs = []
+    for k in range(4):
+        X, Y, _, _ = uniform_cube(5000, seed=40 + k)
+        pairs.append((X, Y))
+    starts = [np.eye(3), rotation([1, 0, 0], 0.4)]
+    Rt0 = np.stack([E.multistart_rt0(a, b, starts) for a, b in pairs])
+    p = E.registrationParameters()
+    try:
+        _lib.check("balance", L.emsa_probe_set_affinity_balance(on))
+        E.set_tiling(0, tile_cols)
+        out = E.emicp_traced(pairs[0][0], pairs[0][1], np.eye(3), np.zeros(3), p) + \
+            E.emicp_batched(pairs, Rt0, p)
+    finally:
+        E.set_tiling(0, 0)
+        _lib.check("balance", L.emsa_probe_set_affinity_balance(1))
+    return out
+
+
+@pytest.mark.parametrize("tile_cols", [2048, 5056])
+def test_affinity_balance_bit_identical(E, tile_cols):
+    """The balanced workgroup order (each step's affinity workgroups sorted by
+    the previous step's work per tile) changes which workgroup computes a
+    tile, not what it computes: a plain traced registration and 8 batched
+    registrations (4 pairs of 5000 points x 2 starts; 2048-column tiles, 3
+    chunks per row block and the row kernel, or whole-row tiles finalized in
+    the affinity kernel; at most 4096 tiles, so the order is in use) give the
+    same bits with the order on and off."""
+    R0, t0, tr0, Rb0, rb0 = _balance_runs(E, 0, tile_cols)
+    R1, t1, tr1, Rb1, rb1 = _balance_runs(E, 1, tile_cols)
+    assert np.array_equal(tr0, tr1)
+    assert np.array_equal(R0, R1) and np.array_equal(t0, t1)
+    assert np.array_equal(Rb0, Rb1) and np.array_equal(rb0, rb1)
+    assert np.all(np.isfinite(Rb1))
+
+
 def test_affinity_cull_non_finite_inputs(E):
     """The device entry point takes clouds as they are. A NaN in the target
     makes its column group's box the whole space and a NaN in the source
