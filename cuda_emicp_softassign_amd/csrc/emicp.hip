@@ -474,10 +474,12 @@ __device__ inline void solve_registration(const EmRegDesc& d, const double* mom,
 // even staged. visits (nullable, measurement): visits[0]
 // receives the number of (valid row, valid column) pairs the launch evaluates,
 // visits[1] those evaluated in the expanded form. perm (nullable): the tile of
-// each workgroup (else tile = workgroup); work (nullable): receives each
+// each workgroup slot (else tile = slot); work (nullable): receives each
 // tile's visited columns, 3 per column of a wave in the expanded form and 4
-// in the difference form (about their relative issue costs), the key the
-// next step's order is sorted by.
+// in the difference form (about their relative issue costs), the key a later
+// order is sorted by. permNext (nullable): workgroup 0 writes the next
+// launch's order there from workPrev (the previous launch's work) and the
+// tiles take slots 1.. (balance_tiles).
 constexpr int kMfPiece = 256;  // columns staged per wave at a time
 // row stride of a wave's stage: 4 floats of padding put the four component
 // rows, which lanes 4b + j read together (row j), in different LDS banks
@@ -486,16 +488,177 @@ constexpr int kMfStageLd = kMfPiece + 4;
 // (-|w|^2 overflows to -inf, and the products with |u|^2 <= xmax stay finite)
 constexpr float kExpFar = 1.0e30f;
 
+// Affinity workgroup order, computed by one more workgroup (kEmRows * kEmWaves
+// threads) of an affinity launch, for the launch after it, in `lds` (the
+// launch's stage, kBalanceLdsBytes or more): the n <= kBalanceMaxTiles tiles
+// in order of their work (one step older than the launch's own), heaviest
+// first, in 64 levels (a counting sort: per level and per group of 64 tiles,
+// the count and each tile's rank among the lanes of its level, found with one
+// ballot per level bit, then prefix sums; equal levels keep the tile order),
+// then dealt to the workgroup slots in rounds of `width` (the compute units;
+// slots b, b + width, ... share one). Up to kGreedyMaxTiles tiles (and width
+// kGreedyMaxWidth) each round gives its heaviest tile to the compute unit with
+// the least work so far (ties: the lower unit), the next to the next, ...;
+// above, a round in even position goes ascending and a full round in odd
+// position descending. perm[b] = the tile of slot b, a permutation of 0..n-1
+// for any work values.
+constexpr int kGreedyMaxTiles = 1280;
+constexpr int kGreedyMaxWidth = 512;
+constexpr int kBalanceLevels = 64;
+constexpr int kBalanceLdsBytes =  // greedy at its limits (the snake deal needs less)
+    ((kGreedyMaxTiles + kEmRows * kEmWaves - 1) / (kEmRows * kEmWaves)) * kEmWaves *
+        kBalanceLevels * 2 +
+    4 * (kEmWaves * kBalanceLevels + kBalanceLevels + kEmWaves) + 4 * kGreedyMaxTiles +
+    8 * kGreedyMaxWidth;
+__device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ perm, int n,
+                              int width, unsigned char* __restrict__ lds) {
+    constexpr int kLevels = kBalanceLevels;
+    constexpr int kThreads = kEmRows * kEmWaves;
+    static_assert(kLevels == kWave && kThreads == kWave * kEmWaves, "one level per lane");
+    static_assert(kBalanceMaxTiles <= 4096, "tile index in 12 bits");
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int rounds = (n + kThreads - 1) / kThreads;  // groups of 64 tiles: rounds * kEmWaves
+    unsigned short* cnt = reinterpret_cast<unsigned short*>(lds);  // [group][level]
+    int* segTot = reinterpret_cast<int*>(lds + (((size_t)rounds * kEmWaves * kLevels * 2 + 15) &
+                                                ~(size_t)15));  // [wave][level]
+    int* lvBase = segTot + kEmWaves * kLevels;  // [level]
+    int* wmax = lvBase + kLevels;               // [wave]
+    int* order = wmax + kEmWaves;               // greedy: (work >> shift) << 12 | tile, heaviest first
+    int* ckey = order + kGreedyMaxTiles;        // greedy: (work so far) << 10 | unit
+    int* pick = ckey + kGreedyMaxWidth;         // greedy: this round's position of each unit
+    const bool greedy = n <= kGreedyMaxTiles && width <= kGreedyMaxWidth;
+    int m = 0;
+    for (int i = tid; i < n; i += kThreads) m = max(m, work[i]);
+    for (int o = 1; o < kWave; o <<= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if (lane == 0) wmax[wave] = m;
+    for (int e = tid; e < rounds * kEmWaves * kLevels; e += kThreads) cnt[e] = 0;
+    __syncthreads();
+    const int mw = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    // level of work w: 63 - floor(w * 64 / (max + 1)) (0: heaviest), in FP32
+    const float scale = (float)kLevels / ((float)mw + 1.f);
+    int shift = 0;  // greedy loads in 16 bits per tile
+    while ((mw >> shift) >= (1 << 16)) ++shift;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // level, and lanes of the wave with the same level (one ballot per bit)
+    auto classify = [&](int r, bool& valid, int& wk, int& lv, unsigned long long& same) {
+        const int i = r * kThreads + tid;
+        valid = i < n;
+        wk = valid ? max(work[i], 0) : 0;
+        lv = kLevels - 1 - min(kLevels - 1, (int)((float)wk * scale));
+        same = __ballot(valid);
+        for (int k = 0; k < 6; ++k) {
+            const unsigned long long b = __ballot(valid && ((lv >> k) & 1));
+            same &= ((lv >> k) & 1) ? b : ~b;
+        }
+    };
+    for (int r = 0; r < rounds; ++r) {  // tiles per (group, level)
+        bool valid;
+        int wk, lv;
+        unsigned long long same;
+        classify(r, valid, wk, lv, same);
+        if (valid && (same & below) == 0)
+            cnt[(r * kEmWaves + wave) * kLevels + lv] = (unsigned short)__popcll(same);
+    }
+    __syncthreads();
+    {  // per level, exclusive prefix over the groups: wave s takes groups [s rounds, (s+1) rounds)
+        int run = 0;
+        for (int k = 0; k < rounds; ++k) {
+            unsigned short* c = cnt + (wave * rounds + k) * kLevels + lane;
+            const int v = *c;
+            *c = (unsigned short)run;
+            run += v;
+        }
+        segTot[wave * kLevels + lane] = run;
+    }
+    __syncthreads();
+    if (wave == 0) {  // the waves' offsets, then the levels' exclusive prefix
+        int t = 0;
+        for (int w2 = 0; w2 < kEmWaves; ++w2) {
+            const int v = segTot[w2 * kLevels + lane];
+            segTot[w2 * kLevels + lane] = t;
+            t += v;
+        }
+        int inc = t;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += v;
+        }
+        lvBase[lane] = inc - t;
+    }
+    __syncthreads();
+    for (int r = 0; r < rounds; ++r) {  // each tile's position
+        bool valid;
+        int wk, lv;
+        unsigned long long same;
+        classify(r, valid, wk, lv, same);
+        if (!valid) continue;
+        const int g = r * kEmWaves + wave;
+        const int pos = lvBase[lv] + segTot[(g / rounds) * kLevels + lv] + cnt[g * kLevels + lv] +
+                        __popcll(same & below);
+        const int i = r * kThreads + tid;
+        if (greedy) {
+            order[pos] = ((wk >> shift) << 12) | i;
+        } else {
+            const int rd = pos / width, p = pos - rd * width;
+            const bool full = (rd + 1) * width <= n;
+            perm[rd * width + (((rd & 1) && full) ? width - 1 - p : p)] = i;
+        }
+    }
+    if (!greedy) return;
+    __syncthreads();
+    for (int c = tid; c < min(width, n); c += kThreads) {  // round 0: unit c, the c-th heaviest
+        const int o = order[c];
+        perm[c] = o & 4095;
+        ckey[c] = ((o >> 12) << 10) | c;
+    }
+    __syncthreads();
+    for (int base = width; base < n; base += width) {
+        const int avail = min(width, n - base);
+        for (int c = tid; c < avail; c += kThreads) {  // rank of unit c by its work so far
+            const int kc = ckey[c];
+            int rank = 0, c2 = 0;
+            for (; c2 + 16 <= avail; c2 += 16) {  // 4 reads in flight
+                int4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const int4*>(ckey + c2 + 4 * u);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    rank += (v[u].x < kc) + (v[u].y < kc) + (v[u].z < kc) + (v[u].w < kc);
+            }
+            for (; c2 < avail; ++c2) rank += (ckey[c2] < kc) ? 1 : 0;
+            pick[c] = rank;
+        }
+        __syncthreads();
+        for (int c = tid; c < avail; c += kThreads) {
+            const int o = order[base + pick[c]];
+            perm[base + c] = o & 4095;
+            ckey[c] = (((ckey[c] >> 10) + (o >> 12)) << 10) | c;
+        }
+        __syncthreads();
+    }
+}
+
 template <bool COMP, int UNR, int NACC, bool FUSE>
 __global__ void __launch_bounds__(kEmRows * kEmWaves) __attribute__((amdgpu_waves_per_eu(8)))
 emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float ka, int cull,
                            float xmax, unsigned long long* __restrict__ visits, double outlier,
                            int last, int iter, const int* __restrict__ perm,
-                           int* __restrict__ work) {
+                           int* __restrict__ work, const int* __restrict__ workPrev,
+                           int* __restrict__ permNext, int ntiles, int width) {
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const int tile = perm ? perm[blockIdx.x] : (int)blockIdx.x;
+    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfStageLd];
+    static_assert(sizeof(stage) >= kBalanceLdsBytes, "the order's sort fits the stage");
+    if (permNext) {  // workgroup 0 orders the next launch's workgroups
+        if (blockIdx.x == 0) {
+            balance_tiles(workPrev, permNext, ntiles, width,
+                          reinterpret_cast<unsigned char*>(&stage[0][0][0]));
+            return;
+        }
+    }
+    const int slot = (int)blockIdx.x - (permNext ? 1 : 0);
+    const int tile = perm ? perm[slot] : slot;
     const int ri = find_reg(regs, nregs, tile);
     const EmRegDesc& d = regs[ri];
     const int M = d.M, ldx = d.ldx, chunks = d.chunks, waveCols = d.waveCols;
@@ -603,7 +766,6 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
         }
     }
 
-    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfStageLd];
     float (*st)[kMfStageLd] = stage[wave];
     // the operand's constant fourth component (B = sum e * 1): the compensated
     // instance writes it once (restaging it per piece kept a constant float4
@@ -865,121 +1027,9 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
 }
 
 
-// The next step's affinity workgroup order (one workgroup of kEmRows *
-// kEmWaves threads): the n <= kBalanceMaxTiles tiles in order of their work,
-// heaviest first, in 64 levels (a counting sort: per level and per group of
-// 64 tiles, the count and each tile's rank among the lanes of its level, then
-// prefix sums; equal levels keep the tile order), dealt to the workgroup
-// slots in rounds of `width` (the compute units): even rounds ascending, odd
-// full rounds descending, so that the slots b, b + width, ... that share a
-// compute unit get heavy and light tiles in turn. perm[b] = the tile of
-// workgroup b, a permutation of 0..n-1 for any work values.
-__device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ perm, int n,
-                              int width) {
-    static_assert(kEmRows * kEmWaves == kWave * kEmWaves, "the solve kernel's workgroup");
-    constexpr int kLevels = 64;
-    constexpr int kThreads = kWave * kEmWaves;
-    constexpr int kRounds = kBalanceMaxTiles / kThreads;  // tiles per thread at most
-    constexpr int kRW = kRounds * kEmWaves;               // (round, wave) slots of 64 tiles
-    constexpr int kSeg = kRW / kEmWaves;                  // slots per scanning wave
-    static_assert(kLevels == kWave, "one level per lane in the scans");
-    __shared__ int cnt[kRW][kLevels];
-    __shared__ int segTot[kEmWaves][kLevels];
-    __shared__ int lvBase[kLevels];
-    __shared__ int wmax[kEmWaves];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    // every load in flight at once: tile r kThreads + tid of each round r
-    int wv[kRounds];
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
-        const int i = r * kThreads + tid;
-        wv[r] = (i < n) ? max(work[i], 0) : 0;
-    }
-    for (int e = tid; e < kRW * kLevels; e += kThreads) (&cnt[0][0])[e] = 0;
-    int m = 0;
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) m = max(m, wv[r]);
-    for (int o = 1; o < kWave; o <<= 1) m = max(m, __shfl_xor(m, o, kWave));
-    if (lane == 0) wmax[wave] = m;
-    __syncthreads();
-    // level of work w: 63 - floor(w * 64 / (max + 1)), in FP32 (an ordering
-    // heuristic: any rounding still gives levels 0..63)
-    const float top = (float)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])) + 1.f;
-    const float scale = (float)kLevels / top;
-    const int rounds = (n + kThreads - 1) / kThreads;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    // per (round, wave): tiles of each level (0: heaviest), and each tile's
-    // rank among the lanes of its level below it (equal levels found by one
-    // ballot per level bit)
-    int lv[kRounds], rk[kRounds];
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
-        lv[r] = 0;
-        rk[r] = 0;
-        if (r >= rounds) continue;  // uniform
-        const bool valid = r * kThreads + tid < n;
-        const int d = kLevels - 1 - min(kLevels - 1, (int)((float)wv[r] * scale));
-        unsigned long long same = __ballot(valid);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const unsigned long long b = __ballot(valid && ((d >> k) & 1));
-            same &= ((d >> k) & 1) ? b : ~b;
-        }
-        lv[r] = d;
-        rk[r] = __popcll(same & below);
-        if (valid && rk[r] == 0) cnt[r * kEmWaves + wave][d] = __popcll(same);
-    }
-    __syncthreads();
-    {  // exclusive prefix over the slots per level: wave s scans slots [s kSeg, (s + 1) kSeg)
-        int c[kSeg];
-#pragma unroll
-        for (int k = 0; k < kSeg; ++k) c[k] = cnt[wave * kSeg + k][lane];
-        int run = 0;
-#pragma unroll
-        for (int k = 0; k < kSeg; ++k) {
-            cnt[wave * kSeg + k][lane] = run;
-            run += c[k];
-        }
-        segTot[wave][lane] = run;
-    }
-    __syncthreads();
-    if (wave == 0) {  // level totals, then their exclusive prefix over the levels
-        int t = 0;
-        for (int s = 0; s < kEmWaves; ++s) {
-            const int c = segTot[s][lane];
-            segTot[s][lane] = t;
-            t += c;
-        }
-        int inc = t;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int v = __shfl_up(inc, o, kWave);
-            if (lane >= o) inc += v;
-        }
-        lvBase[lane] = inc - t;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
-        const int i = r * kThreads + tid;
-        if (i < n) {
-            const int rw = r * kEmWaves + wave, d = lv[r];
-            const int pos = lvBase[d] + segTot[rw / kSeg][d] + cnt[rw][d] + rk[r];
-            const int round = pos / width, p = pos - round * width;
-            const bool full = (round + 1) * width <= n;
-            perm[round * width + (((round & 1) && full) ? width - 1 - p : p)] = i;
-        }
-    }
-}
-
-// Stage 3: one workgroup per registration; with work != nullptr one more
-// workgroup (the last) orders the next step's affinity workgroups.
+// Stage 3: one workgroup per registration.
 __global__ void __launch_bounds__(kEmRows * kEmWaves)
-emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter, int nregs,
-                   const int* __restrict__ work, int* __restrict__ perm, int ntiles, int width) {
-    if ((int)blockIdx.x >= nregs) {
-        balance_tiles(work, perm, ntiles, width);
-        return;
-    }
+emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter) {
     const EmRegDesc& d = regs[blockIdx.x];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1019,7 +1069,8 @@ emicp_solve_kernel(const EmRegDesc* __restrict__ regs, float sigma_p2, int iter,
 // the all-VALU kernel.
 void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, unsigned tiles,
                      float sigma_p2, hipStream_t s, unsigned long long* visits,
-                     const EmRowFuse* fuse, const int* perm = nullptr, int* work = nullptr) {
+                     const EmRowFuse* fuse, const int* perm = nullptr, int* work = nullptr,
+                     const int* workPrev = nullptr, int* permNext = nullptr) {
     const dim3 blk(kEmRows * kEmWaves);
     const bool comp = sigma_p2 < kCompBelowSigma2;
     if (ctx.emAffinityMfma) {
@@ -1027,14 +1078,15 @@ void launch_affinity(const DeviceCtx& ctx, const EmRegDesc* desc, int nregs, uns
         const float ka = (float)std::sqrt(kLog2e / (2.0 * (double)sigma_p2));
         const double outlier = fuse ? fuse->outlier : 0.0;
         const int last = fuse ? fuse->last : 0, iter = fuse ? fuse->iter : 0;
+        const unsigned grid = tiles + (permNext ? 1u : 0u);
 // 2 matrix accumulators: with 4, the instances that finalize rows in their
 // epilogue held one register pair in scratch between the row setup and the
 // epilogue (2.6 MB written per batched 8 x 10k launch); 2 give the same launch
 // times for the batched (whole-row and 2048-column tiles) and plain runs
 #define EMSA_AFF(C_, F_)                                                                     \
-    hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 2, F_>), dim3(tiles), blk, 0, s, desc, \
+    hipLaunchKernelGGL((emicp_affinity_mfma_kernel<C_, 8, 2, F_>), dim3(grid), blk, 0, s, desc, \
                        nregs, ka, ctx.emCull, ctx.emExpanded, visits, outlier, last, iter, perm, \
-                       work)
+                       work, workPrev, permNext, (int)tiles, std::max(1, ctx.numCU))
         if (fuse) {
             if (comp) EMSA_AFF(true, true);
             else EMSA_AFF(false, true);
@@ -1222,7 +1274,7 @@ int em_prepare(DeviceCtx& ctx, EmWork& w, const EmPair* pairs, int count, int n_
     for (const EmRegDesc& d : w.regs) launchPairs += (double)d.N * (double)d.M;
     w.balance = ctx.emBalance && ctx.emAffinityMfma && ctx.emCull && !w.ordOff.empty() &&
                 w.tiles <= kBalanceMaxTiles && launchPairs >= kBalanceMinPairs;
-    if (w.balance && (rc = w.sched.reserve(2 * sizeof(int) * (size_t)w.tiles)) != EMSA_OK)
+    if (w.balance && (rc = w.sched.reserve(4 * sizeof(int) * (size_t)w.tiles)) != EMSA_OK)
         return rc;
     em_bind(w, w.xpad.as<float>(), w.partial.as<float>(), w.moments.as<double>(),
             w.state.as<double>(), sz.ordW > 0 ? w.order.as<unsigned>() : nullptr);
@@ -1245,19 +1297,26 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
     // others and is not launched when there are none
     const bool fuse = ctx.emRowFuse && ctx.emAffinityMfma && w.someSingleChunk;
     const bool rowsKernel = !(fuse && w.singleChunk);
-    // balanced workgroup order: step 0 in tile order, each later step in the
-    // order the previous step's solve launch sorted from its work
-    int* const work = w.balance ? w.sched.as<int>() : nullptr;
-    int* const perm = w.balance ? w.sched.as<int>() + w.tiles : nullptr;
+    // balanced workgroup order: steps 0 and 1 in tile order; the affinity
+    // launch of step k >= 1 also orders step k + 1 from step k - 1's work
+    // (work and orders double-buffered: [work even, work odd, order even,
+    // order odd] steps)
+    int* const sched = w.balance ? w.sched.as<int>() : nullptr;
+    const size_t T = (size_t)w.tiles;
     for (size_t it = 0; it < sigmas.size(); ++it) {
         const float sig = sigmas[it];
         const double outlier = (double)expf(-p.d_02 / sig);  // emicp.cu:489, host float
         const int last = (it + 1 == sigmas.size()) ? 1 : 0;
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it], s));
         const EmRowFuse fz{outlier, last, (int)it};
+        const bool orderNext = sched && it >= 1 && it + 1 < sigmas.size();
         launch_affinity(ctx, desc, nregs, (unsigned)w.tiles, sig, s,
                         (probe && probe->visits) ? probe->visits + 2 * it : nullptr,
-                        fuse ? &fz : nullptr, it > 0 ? perm : nullptr, work);
+                        fuse ? &fz : nullptr,
+                        (sched && it >= 2) ? sched + (2 + it % 2) * T : nullptr,
+                        sched ? sched + (it % 2) * T : nullptr,
+                        orderNext ? sched + ((it - 1) % 2) * T : nullptr,
+                        orderNext ? sched + (2 + (it + 1) % 2) * T : nullptr);
         EMSA_HIP_TRY(hipGetLastError());
         if (probe && probe->events) EMSA_HIP_TRY(hipEventRecord(probe->events[2 * it + 1], s));
         if (rowsKernel) {
@@ -1265,10 +1324,8 @@ int em_enqueue(DeviceCtx& ctx, EmWork& w, const float* dRt0, float* dRtOut, doub
                                desc, nregs, w.rowBlocks, outlier, last, (int)it, fuse ? 1 : 0);
             EMSA_HIP_TRY(hipGetLastError());
         }
-        const bool order = work && !last;  // the next step's workgroup order
-        hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs + (order ? 1 : 0)),
-                           dim3(kEmRows * kEmWaves), 0, s, desc, sig, (int)it, nregs, work, perm,
-                           (int)w.tiles, std::max(1, ctx.numCU));
+        hipLaunchKernelGGL(emicp_solve_kernel, dim3(nregs), dim3(kEmRows * kEmWaves), 0, s, desc,
+                           sig, (int)it);
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(emicp_output_kernel, dim3(nregs), dim3(64), 0, s, desc, dRtOut, dResid);

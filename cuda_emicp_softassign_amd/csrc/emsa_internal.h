@@ -124,23 +124,23 @@ struct EmPair {
 
 // Affinity workgroup order across sigma^2 steps: each step's affinity launch
 // writes the work of every tile (its visited columns, weighted by the form of
-// the exponent) and the step's solve launch sorts the tiles by it for the next
-// step, heaviest first, dealt to the workgroup slots so that the ones that
-// share a compute unit (the hardware places workgroups b and b + numCU on the
-// same one while all are resident) add up to about the same work. Used when
-// the launches skip column groups (ordered clouds), so that the work per tile
-// differs, for at most kBalanceMaxTiles tiles (the sort, one workgroup's, then
-// hides behind the solve: 1256 tiles ~1.6 us, 2048 ~2.9 us) and at least
+// the exponent) and one more workgroup of the next launch sorts the tiles by
+// it for the launch after that (so the sort runs beside the tiles, off the
+// critical path), heaviest first, dealt to the workgroup slots so that the
+// ones that share a compute unit (the hardware places workgroups b and
+// b + numCU on the same one while all are resident) add up to about the same
+// work. Used when the launches skip column groups (ordered clouds), so that
+// the work per tile differs, for at most kBalanceMaxTiles tiles and at least
 // kBalanceMinPairs pairs per launch (below, a launch is too short for the
-// extra load of the order to pay). The order changes which workgroup computes
-// a tile, never what it computes.
+// extra load of the order to pay). The order changes which workgroup
+// computes a tile, never what it computes.
 constexpr int kBalanceMaxTiles = 2048;
 constexpr double kBalanceMinPairs = 16777216.0;
 
 // Workspace and descriptor table of one EM-ICP batch.
 struct EmWork {
     DevBuf xpad, partial, moments, state, desc, order;
-    DevBuf sched;  // balanced order: work [tiles], then the tile of each workgroup [tiles]
+    DevBuf sched;  // balanced order: work [2][tiles], then the tile of each slot [2][tiles]
     bool balance = false;  // sched is sized and the affinity launches use it
     std::vector<EmRegDesc> regs;
     std::vector<size_t> ordOff;  // per registration: its pair's offset in `order` (words);
