@@ -140,8 +140,8 @@ struct SaBufs {
 // inner iteration with fresh column potentials (g = 0): P, and the column
 // terms of the matrix-core row passes for exponent constants (k1, k0).
 __device__ void sa_prep_columns(const SaBufs& b, int M, int ldk, float k1, float k0,
-                                const double* Rm, const double* tm) {
-    for (int c = threadIdx.x; c < ldk; c += blockDim.x) {
+                                const double* Rm, const double* tm, int c0, int c1) {
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
         float p[3];
         if (c < M) {
             const double y0 = b.Y[c], y1 = b.Y[(size_t)M + c], y2 = b.Y[2 * (size_t)M + c];
@@ -178,7 +178,7 @@ sa_setup_kernel(SaBufs b, int N, int M, int ldk, const float* __restrict__ Rt, f
         double Rm[9], tm[3];
         for (int i = 0; i < 9; ++i) Rm[i] = (double)Rt[i];
         for (int i = 0; i < 3; ++i) tm[i] = (double)Rt[9 + i];
-        sa_prep_columns(b, M, ldk, k1, k0, Rm, tm);
+        sa_prep_columns(b, M, ldk, k1, k0, Rm, tm, 0, ldk);
     }
 }
 
@@ -697,9 +697,12 @@ sa_colfin_kernel(SaBufs b, int M, int ldk, int H, int last) {
         b.cby[(size_t)lane * gridDim.x + blockIdx.x] = (lane == 0) ? by[0] : (lane == 1 ? by[1] : by[2]);
 }
 
-// One workgroup: totals of the per-workgroup moments and of sum b y, S, Horn
-// and the new (R, t) in the state. The next inner iteration's first row pass
-// transforms the sources with it. Wave w sums quantities q = w % 4 + 4 k over
+// Totals of the per-workgroup moments and of sum b y, S, Horn and the new
+// (R, t) in the state (workgroup 0). With prep, every workgroup of the launch
+// computes the same totals and (R, t) (same loads, same order: the same bits)
+// and prepares its own slice of kWave * kSaSolveWaves columns for the next
+// inner iteration's matrix-core passes; without, one workgroup, and the next
+// inner iteration's first row pass transforms the sources itself. Wave w sums quantities q = w % 4 + 4 k over
 // the (w / 4)-th of kSaSolveWaves / 4 ranges of the partials (lanes striding,
 // four partial sums per lane so the loads are in flight together), then a
 // wave sum; the ranges are added in a fixed order in LDS.
@@ -718,48 +721,54 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter, int M, int ldk,
     const int len = (nmom + kParts - 1) / kParts;
     const int wbeg = part * len, wend = min(wbeg + len, nmom);
     // rows up to kSaMomPad - 1 exist (the ones past kSaMomW are not used):
-    // no per-quantity branch
+    // no per-quantity branch. Each lane loads kU partials of each of its kPer
+    // quantities in one round (predicated, all in flight together: one round
+    // trip for up to kU * 64 partials per range), then adds them in a fixed
+    // order.
     constexpr int kPer = kSaMomPad / kSaWaves;
-    constexpr int kU = 4;
-    static_assert(kU == 4, "the final combine adds four partial sums");
-    double acc[kPer][kU];
+    constexpr int kU = 5;
+    double acc[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
+    for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+    for (int w0 = wbeg; w0 < wend; w0 += kU * kWave) {
+        double v[kU][kPer];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) acc[k][u] = 0.0;
-    int w = wbeg + lane;
-    for (; w + (kU - 1) * kWave < wend; w += kU * kWave)
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
+        for (int u = 0; u < kU; ++u) {
+            const int w = w0 + u * kWave + lane;
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
-                acc[k][u] += b.rmom[(size_t)(qc + kSaWaves * k) * nmom + w + u * kWave];
-    for (; w < wend; w += kWave)
+                v[u][k] = (w < wend) ? b.rmom[(size_t)(qc + kSaWaves * k) * nmom + w] : 0.0;
+        }
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) acc[k][0] += b.rmom[(size_t)(qc + kSaWaves * k) * nmom + w];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) acc[k][0] = (acc[k][0] + acc[k][1]) + (acc[k][2] + acc[k][3]);
-    const bool byWave = (part == 0 && qc < 3);  // component qc of sum b y
+        for (int k = 0; k < kPer; ++k)
+            acc[k] += ((v[0][k] + v[1][k]) + (v[2][k] + v[3][k])) + v[4][k];
+    }
+    // component qc < 3 of sum b y: its partials split over the kParts ranges
+    // like the moments
     double by = 0.0;
-    if (byWave) {  // four partial sums per lane so the loads are in flight together
-        double bq[4] = {0.0, 0.0, 0.0, 0.0};
-        int c = lane;
-        for (; c + 3 * kWave < nfin; c += 4 * kWave)
+    if (qc < 3) {
+        const int blen = (nfin + kParts - 1) / kParts;
+        const int bbeg = part * blen, bend = min(bbeg + blen, nfin);
+        for (int c0 = bbeg; c0 < bend; c0 += kU * kWave) {
+            double v[kU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) bq[u] += b.cby[(size_t)qc * nfin + c + u * kWave];
-        for (; c < nfin; c += kWave) bq[0] += b.cby[(size_t)qc * nfin + c];
-        by = (bq[0] + bq[1]) + (bq[2] + bq[3]);
+            for (int u = 0; u < kU; ++u) {
+                const int c = c0 + u * kWave + lane;
+                v[u] = (c < bend) ? b.cby[(size_t)qc * nfin + c] : 0.0;
+            }
+            by += ((v[0] + v[1]) + (v[2] + v[3])) + v[4];
+        }
     }
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) acc[k][0] = wave_sum(acc[k][0]);
+    for (int k = 0; k < kPer; ++k) acc[k] = wave_sum(acc[k]);
     by = wave_sum(by);
     __shared__ double red[kParts][kSaMomPad];
-    __shared__ double rby[3];
+    __shared__ double rby[kParts][3];
     __shared__ double mt[kSaMomW + 3];
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) red[part][qc + kSaWaves * k] = acc[k][0];
-        if (byWave) rby[qc] = by;
+        for (int k = 0; k < kPer; ++k) red[part][qc + kSaWaves * k] = acc[k];
+        if (qc < 3) rby[part][qc] = by;
     }
     __syncthreads();
     if (tid < kSaMomW) {
@@ -768,9 +777,13 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter, int M, int ldk,
         for (int pp = 1; pp < kParts; ++pp) t += red[pp][tid];
         mt[tid] = t;
     } else if (tid < kSaMomW + 3) {
-        mt[tid] = rby[tid - kSaMomW];
+        double t = rby[0][tid - kSaMomW];
+#pragma unroll
+        for (int pp = 1; pp < kParts; ++pp) t += rby[pp][tid - kSaMomW];
+        mt[tid] = t;
     }
     __syncthreads();
+    __shared__ double rtNew[12];  // (R, t) after this solve, for the column slice
     if (tid == 0) {
         const double Sg = mt[0];
         double Xc[3], Yc[3], S[9], Rn[9], tn[3];
@@ -784,15 +797,20 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter, int M, int ldk,
                 S[i + 3 * j] = mt[4 + 3 * i + j] - Yc[j] * mt[13 + i] - Xc[i] * mt[16 + j] +
                                Xc[i] * Yc[j] * Sg;
         const bool ok = horn_rt(Xc, Yc, S, Rn, tn);
+        const bool owner = blockIdx.x == 0;  // the one that writes the state and trace
         if (ok) {
-            for (int i = 0; i < 9; ++i) b.state[kStR + i] = Rn[i];
-            for (int i = 0; i < 3; ++i) b.state[kStT + i] = tn[i];
-        } else {
-            b.state[kStStatus] += 1.0;
+            if (owner) {
+                for (int i = 0; i < 9; ++i) b.state[kStR + i] = Rn[i];
+                for (int i = 0; i < 3; ++i) b.state[kStT + i] = tn[i];
+            }
+        } else {  // (R, t) stay: no workgroup writes them on this path
+            if (owner) b.state[kStStatus] += 1.0;
             for (int i = 0; i < 9; ++i) Rn[i] = b.state[kStR + i];
             for (int i = 0; i < 3; ++i) tn[i] = b.state[kStT + i];
         }
-        if (b.trace) {
+        for (int i = 0; i < 9; ++i) rtNew[i] = Rn[i];
+        for (int i = 0; i < 3; ++i) rtNew[9 + i] = tn[i];
+        if (b.trace && owner) {
             double* o = b.trace + (size_t)iter * EMSA_TRACE_W;
             o[0] = T;
             o[1] = Sg;
@@ -813,9 +831,11 @@ sa_solve_kernel(SaBufs b, int nmom, int nfin, float T, int iter, int M, int ldk,
     if (prep) {
         __syncthreads();
         double Rm[9], tm[3];
-        for (int i = 0; i < 9; ++i) Rm[i] = b.state[kStR + i];
-        for (int i = 0; i < 3; ++i) tm[i] = b.state[kStT + i];
-        sa_prep_columns(b, M, ldk, k1n, k0n, Rm, tm);
+        for (int i = 0; i < 9; ++i) Rm[i] = rtNew[i];
+        for (int i = 0; i < 3; ++i) tm[i] = rtNew[9 + i];
+        const int per = kWave * kSaSolveWaves;
+        const int c0 = (int)blockIdx.x * per;
+        sa_prep_columns(b, M, ldk, k1n, k0n, Rm, tm, c0, min(ldk, c0 + per));
     }
 }
 
@@ -1265,7 +1285,11 @@ int softassign_device(DeviceCtx& ctx, const float* dX, int N, const float* dY, i
             float k1n = 0.f, k0n = 0.f;
             bool expn = false;
             if (more) consts(Tn, k1n, k0n, expn);
-            hipLaunchKernelGGL(sa_solve_kernel, dim3(1), dim3(kWave * kSaSolveWaves), 0, s, b,
+            // with the columns to prepare, one workgroup per kWave * kSaSolveWaves of them
+            const unsigned sgrid = expn ? (unsigned)((pl.ldk + kWave * kSaSolveWaves - 1) /
+                                                     (kWave * kSaSolveWaves))
+                                        : 1u;
+            hipLaunchKernelGGL(sa_solve_kernel, dim3(sgrid), dim3(kWave * kSaSolveWaves), 0, s, b,
                                (LOGV && expand) ? mfRows : nmom,
                                (LOGV && expand) ? mfCols : finBlocks, T, it, M, pl.ldk,
                                expn ? 1 : 0, k1n, k0n);
