@@ -554,46 +554,84 @@ adj_rows_kernel(const AdjDesc* __restrict__ regs, int nregs, int k, double gsig_
     if (threadIdx.x < kAdjMomW) d.rtmom[(size_t)rt * kAdjMomW + threadIdx.x] = tot;
 }
 
-// Stage 4: one thread per target column, which also finishes the step for
-// its registration: the row tiles' moments summed in tile order give the
-// floored rows' dL/dV sum (every column thread sums these 3 itself) and the
-// next step's dL/dR, dL/dt and the dL/dsigma_0^2 term (written by the
-// registration's column-0 thread: no other thread reads the state here).
-// fk = factor^k. A step whose Horn solve failed in the forward left (R, t)
-// unchanged, so its gradient passes through.
+constexpr int kAdjTileRound = 8;  // row tiles whose partials a thread loads at once
+
+// Stage 4: one thread per target column (workgroups 0 .. colBlocks - 1), and
+// one workgroup per registration after them that finishes the step: the row
+// tiles' moments summed in tile order (threads 0..15, one moment each) give
+// the next step's dL/dR, dL/dt, the dL/dsigma_0^2 term and the floored rows'
+// dL/dV sum, then Horn's step backwards for the next (earlier) step (its
+// coefficients for the pair pass that follows; the pair pass of this step has
+// read the current ones). Every column thread sums the floored rows' dL/dV
+// itself. Sums over the row tiles load kAdjTileRound tiles at once
+// (predicated) and add them in tile order. fk = factor^k. A step whose Horn
+// solve failed in the forward left (R, t) unchanged, so its gradient passes
+// through.
 __global__ void __launch_bounds__(256)
 adj_cols_kernel(const AdjDesc* __restrict__ regs, int nregs, long long totalCols, int k,
-                double fk, double inv_ka, int next) {
+                double fk, double inv_ka, int next, int colBlocks) {
+    static_assert(kAdjMomW >= 16, "moments 0..12 and the floored dL/dV 13..15");
+    if ((int)blockIdx.x >= colBlocks) {
+        const AdjDesc& d = regs[blockIdx.x - colBlocks];
+        __shared__ double msum[16];
+        const int q = threadIdx.x, rowTiles = d.rowTiles;
+        if (q < 16) {
+            double a = 0.0;
+            for (int t0 = 0; t0 < rowTiles; t0 += kAdjTileRound) {
+                double v[kAdjTileRound];
+#pragma unroll
+                for (int u = 0; u < kAdjTileRound; ++u)
+                    v[u] = (t0 + u < rowTiles) ? d.rtmom[(size_t)(t0 + u) * kAdjMomW + q] : 0.0;
+#pragma unroll
+                for (int u = 0; u < kAdjTileRound; ++u) a += v[u];
+            }
+            msum[q] = a;
+        }
+        __syncthreads();
+        if (q == 0) {
+            const bool failed = d.rec[(size_t)k * EMSA_TRACE_W + 31] != 0.0;
+            if (!failed)
+                for (int i = 0; i < 12; ++i) d.adj[i] = msum[i];  // dL/dR_k, dL/dt_k
+            d.adj[12] += fk * msum[12];                           // dL/dsigma_0^2
+            for (int i = 0; i < 3; ++i) d.adj[13 + i] = msum[13 + i];
+            if (next >= 0)
+                horn_backward(d.rec + (size_t)next * EMSA_TRACE_W, d.adj, d.adj + 9, d.coef);
+        }
+        return;
+    }
     const long long gc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gc >= totalCols) return;
     const int ri = find_by(regs, nregs, (int)gc, [](const AdjDesc& a) { return a.colBase; });
     const AdjDesc& d = regs[ri];
     const int c = (int)gc - d.colBase;
     if (c >= d.N) return;
+    const int rowTiles = d.rowTiles;
     double gfl[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < d.rowTiles; ++t)
-        for (int i = 0; i < 3; ++i) gfl[i] += d.rtmom[(size_t)t * kAdjMomW + 13 + i];
-    if (c == 0) {
-        const bool failed = d.rec[(size_t)k * EMSA_TRACE_W + 31] != 0.0;
-        double m[13];
-        for (int i = 0; i < 13; ++i) m[i] = 0.0;
-        for (int t = 0; t < d.rowTiles; ++t)
-            for (int i = 0; i < 13; ++i) m[i] += d.rtmom[(size_t)t * kAdjMomW + i];
-        if (!failed)
-            for (int i = 0; i < 12; ++i) d.adj[i] = m[i];  // dL/dR_k, dL/dt_k
-        d.adj[12] += fk * m[12];                          // dL/dsigma_0^2
-        for (int i = 0; i < 3; ++i) d.adj[13 + i] = gfl[i];
-        // Horn's step backwards for the next (earlier) step: its coefficients
-        // for the pair pass that follows (the pair pass of this step has read
-        // the current ones)
-        if (next >= 0) horn_backward(d.rec + (size_t)next * EMSA_TRACE_W, d.adj, d.adj + 9, d.coef);
+    for (int t0 = 0; t0 < rowTiles; t0 += kAdjTileRound) {
+        double v[kAdjTileRound][3];
+#pragma unroll
+        for (int u = 0; u < kAdjTileRound; ++u)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                v[u][i] = (t0 + u < rowTiles) ? d.rtmom[(size_t)(t0 + u) * kAdjMomW + 13 + i] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kAdjTileRound; ++u)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) gfl[i] += v[u][i];
     }
     double g[3] = {0.0, 0.0, 0.0};  // the pair terms, ka times dL/dx (see adj_pairs_kernel)
-    for (int t = 0; t < d.rowTiles; ++t) {
-        const float* o = d.colpart + ((size_t)t * d.ldx + c) * 3;
-        g[0] += o[0];
-        g[1] += o[1];
-        g[2] += o[2];
+    for (int t0 = 0; t0 < rowTiles; t0 += kAdjTileRound) {
+        float v[kAdjTileRound][3];
+#pragma unroll
+        for (int u = 0; u < kAdjTileRound; ++u) {
+            const float* o = d.colpart + ((size_t)(t0 + u) * d.ldx + c) * 3;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[u][i] = (t0 + u < rowTiles) ? o[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kAdjTileRound; ++u)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g[i] += v[u][i];
     }
     for (int i = 0; i < 3; ++i) g[i] = g[i] * inv_ka + d.floorA * gfl[i];
     d.gX[c] += (float)g[0];
@@ -798,8 +836,9 @@ int emsa_emicp_unrolled_bwd(const emsa_pair* pairs, int count, const float* dRt0
                                dim3(kWave * kAdjWaves), 0, s, desc, count, k, ka, minv);
         hipLaunchKernelGGL(adj_rows_kernel, dim3((unsigned)rowTiles), dim3(kAdjRows), 0, s, desc,
                            count, k, gsig_scale, outlier_dsig, inv_ka);
-        hipLaunchKernelGGL(adj_cols_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
-                           desc, count, cols, k, fk, inv_ka, k - 1);
+        const unsigned colBlocks = (unsigned)((cols + 255) / 256);  // then one per registration
+        hipLaunchKernelGGL(adj_cols_kernel, dim3(colBlocks + (unsigned)count), dim3(256), 0, s,
+                           desc, count, cols, k, fk, inv_ka, k - 1, (int)colBlocks);
         EMSA_HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(adj_output_kernel, dim3(count), dim3(64), 0, s, desc, dgRt0, dgsigma);
