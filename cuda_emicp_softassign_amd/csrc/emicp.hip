@@ -505,11 +505,18 @@ constexpr float kExpFar = 1.0e30f;
 constexpr int kGreedyMaxTiles = 1280;
 constexpr int kGreedyMaxWidth = 512;
 constexpr int kBalanceLevels = 64;
-constexpr int kBalanceLdsBytes =  // greedy at its limits (the snake deal needs less)
-    ((kGreedyMaxTiles + kEmRows * kEmWaves - 1) / (kEmRows * kEmWaves)) * kEmWaves *
-        kBalanceLevels * 2 +
-    4 * (kEmWaves * kBalanceLevels + kBalanceLevels + kEmWaves) + 4 * kGreedyMaxTiles +
-    8 * kGreedyMaxWidth;
+// LDS of the sort for n tiles: counts (2 bytes per group of 64 tiles and
+// level), the tiles' levels (1 byte each), the prefix sums, and for the
+// greedy deal the sorted tiles and the units' keys and picks
+constexpr int balance_lds_bytes(int n, bool greedy) {
+    return ((n + kEmRows * kEmWaves - 1) / (kEmRows * kEmWaves)) * kEmWaves * kBalanceLevels * 2 +
+           (n + 15) / 16 * 16 + 4 * (kEmWaves * kBalanceLevels + kBalanceLevels + kEmWaves) +
+           (greedy ? 4 * kGreedyMaxTiles + 8 * kGreedyMaxWidth : 0);
+}
+constexpr int kBalanceLdsBytes = balance_lds_bytes(kBalanceMaxTiles, false) >
+                                         balance_lds_bytes(kGreedyMaxTiles, true)
+                                     ? balance_lds_bytes(kBalanceMaxTiles, false)
+                                     : balance_lds_bytes(kGreedyMaxTiles, true);
 __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ perm, int n,
                               int width, unsigned char* __restrict__ lds) {
     constexpr int kLevels = kBalanceLevels;
@@ -519,8 +526,8 @@ __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ pe
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int rounds = (n + kThreads - 1) / kThreads;  // groups of 64 tiles: rounds * kEmWaves
     unsigned short* cnt = reinterpret_cast<unsigned short*>(lds);  // [group][level]
-    int* segTot = reinterpret_cast<int*>(lds + (((size_t)rounds * kEmWaves * kLevels * 2 + 15) &
-                                                ~(size_t)15));  // [wave][level]
+    unsigned char* lvl = lds + (size_t)rounds * kEmWaves * kLevels * 2;  // [tile] its level
+    int* segTot = reinterpret_cast<int*>(lvl + (n + 15) / 16 * 16);     // [wave][level]
     int* lvBase = segTot + kEmWaves * kLevels;  // [level]
     int* wmax = lvBase + kLevels;               // [wave]
     int* order = wmax + kEmWaves;               // greedy: (work >> shift) << 12 | tile, heaviest first
@@ -528,6 +535,7 @@ __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ pe
     int* pick = ckey + kGreedyMaxWidth;         // greedy: this round's position of each unit
     const bool greedy = n <= kGreedyMaxTiles && width <= kGreedyMaxWidth;
     int m = 0;
+#pragma unroll 4
     for (int i = tid; i < n; i += kThreads) m = max(m, work[i]);
     for (int o = 1; o < kWave; o <<= 1) m = max(m, __shfl_xor(m, o, kWave));
     if (lane == 0) wmax[wave] = m;
@@ -538,13 +546,17 @@ __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ pe
     const float scale = (float)kLevels / ((float)mw + 1.f);
     int shift = 0;  // greedy loads in 16 bits per tile
     while ((mw >> shift) >= (1 << 16)) ++shift;
+#pragma unroll 4
+    for (int i = tid; i < n; i += kThreads)  // each tile's level, into LDS
+        lvl[i] = (unsigned char)(kLevels - 1 -
+                                 min(kLevels - 1, (int)((float)max(work[i], 0) * scale)));
+    __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
     // level, and lanes of the wave with the same level (one ballot per bit)
-    auto classify = [&](int r, bool& valid, int& wk, int& lv, unsigned long long& same) {
+    auto classify = [&](int r, bool& valid, int& lv, unsigned long long& same) {
         const int i = r * kThreads + tid;
         valid = i < n;
-        wk = valid ? max(work[i], 0) : 0;
-        lv = kLevels - 1 - min(kLevels - 1, (int)((float)wk * scale));
+        lv = valid ? (int)lvl[i] : 0;
         same = __ballot(valid);
         for (int k = 0; k < 6; ++k) {
             const unsigned long long b = __ballot(valid && ((lv >> k) & 1));
@@ -553,9 +565,9 @@ __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ pe
     };
     for (int r = 0; r < rounds; ++r) {  // tiles per (group, level)
         bool valid;
-        int wk, lv;
+        int lv;
         unsigned long long same;
-        classify(r, valid, wk, lv, same);
+        classify(r, valid, lv, same);
         if (valid && (same & below) == 0)
             cnt[(r * kEmWaves + wave) * kLevels + lv] = (unsigned short)__popcll(same);
     }
@@ -588,16 +600,16 @@ __device__ void balance_tiles(const int* __restrict__ work, int* __restrict__ pe
     __syncthreads();
     for (int r = 0; r < rounds; ++r) {  // each tile's position
         bool valid;
-        int wk, lv;
+        int lv;
         unsigned long long same;
-        classify(r, valid, wk, lv, same);
+        classify(r, valid, lv, same);
         if (!valid) continue;
         const int g = r * kEmWaves + wave;
         const int pos = lvBase[lv] + segTot[(g / rounds) * kLevels + lv] + cnt[g * kLevels + lv] +
                         __popcll(same & below);
         const int i = r * kThreads + tid;
         if (greedy) {
-            order[pos] = ((wk >> shift) << 12) | i;
+            order[pos] = ((max(work[i], 0) >> shift) << 12) | i;
         } else {
             const int rd = pos / width, p = pos - rd * width;
             const bool full = (rd + 1) * width <= n;

@@ -134,7 +134,7 @@ struct EmPair {
 // kBalanceMinPairs pairs per launch (below, a launch is too short for the
 // extra load of the order to pay). The order changes which workgroup
 // computes a tile, never what it computes.
-constexpr int kBalanceMaxTiles = 2048;
+constexpr int kBalanceMaxTiles = 4096;
 constexpr double kBalanceMinPairs = 16777216.0;
 
 // Workspace and descriptor table of one EM-ICP batch.

@@ -97,10 +97,10 @@ int emsa_probe_set_row_fuse(int enable);
 
 /* Selects, on the current device, the order of the EM-ICP affinity
  * workgroups of launches that skip column groups (ordered clouds) and have at
- * most 2048 tiles and at least 2^24 pairs: 1 (default) from the second
- * sigma^2 step on, sorted by the previous step's work per tile and dealt so
+ * most 4096 tiles and at least 2^24 pairs: 1 (default) from the third
+ * sigma^2 step on, sorted by an earlier step's work per tile and dealt so
  * that the workgroups sharing a compute unit add up to about the same work
- * (the previous step's solve launch sorts them); 0 in tile order. Which
+ * (one more workgroup of the previous launch sorts them); 0 in tile order. Which
  * workgroup computes a tile changes, what it computes does not: results are
  * bit-identical. For A/B measurement; drops the recorded launch sequences. */
 int emsa_probe_set_affinity_balance(int enable);
