@@ -1123,9 +1123,12 @@ EmPlan plan_emicp(int N, int M, int numCU, int blocks_per_cu) {
     pl.rowBlocks = (M + kEmRows - 1) / kEmRows;
     pl.ldx = (N + kColGroup - 1) / kColGroup * kColGroup;
     const int groups = pl.ldx / kColGroup;  // column groups of 16
-    // 12 workgroups per CU: measured best for a plain 10k registration (more
-    // chunks speed up the affinity but lengthen the row finalize)
-    const int target = std::max(1, numCU) * (blocks_per_cu > 0 ? blocks_per_cu : 12);
+    // workgroups per CU: 7 where the launches run in the balanced order (at
+    // least kBalanceMinPairs pairs; plain 10k registration 2.08 ms against
+    // 2.24 ms with 12), else 12 (N = M = 1007: 0.92 ms against 0.93 with 6
+    // to 8): more chunks speed up the affinity but lengthen the row finalize
+    const int dflt = ((double)N * (double)M >= kBalanceMinPairs) ? 7 : 12;
+    const int target = std::max(1, numCU) * (blocks_per_cu > 0 ? blocks_per_cu : dflt);
     int chunks = (target + pl.rowBlocks - 1) / pl.rowBlocks;
     chunks = std::max(1, std::min(chunks, (groups + kEmWaves - 1) / kEmWaves));
     const int gpw = (groups + chunks * kEmWaves - 1) / (chunks * kEmWaves);  // groups per wave
