@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # per source: the Softassign log-domain passes chain short matrix-core
 # products into VALU work; their results go straight to VGPRs
-SOURCE_FLAGS = {"softassign.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+SOURCE_FLAGS = {"softassign.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+                "emicp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall",
           "-Wno-unused-function", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

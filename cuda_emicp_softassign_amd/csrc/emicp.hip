@@ -484,9 +484,20 @@ constexpr int kMfPiece = 256;  // columns staged per wave at a time
 // row stride of a wave's stage: 4 floats of padding put the four component
 // rows, which lanes 4b + j read together (row j), in different LDS banks
 constexpr int kMfStageLd = kMfPiece + 4;
+// stage rows: x, y, z (or w), the constant 1, and (expanded form) -|w|^2
+constexpr int kMfStageRows = 5;
 // EXP stage value of a slot the piece does not fill: exp2 of its exponent is 0
 // (-|w|^2 overflows to -inf, and the products with |u|^2 <= xmax stay finite)
 constexpr float kExpFar = 1.0e30f;
+
+// -|w|^2 of 4 staged columns (the signs on the FMA operands), formed once per
+// piece: the accumulator init of the expanded exponent
+__device__ inline float4 neg_sq_norm4(float4 x, float4 y, float4 z) {
+    return make_float4(fmaf(-x.x, x.x, fmaf(-y.x, y.x, -z.x * z.x)),
+                       fmaf(-x.y, x.y, fmaf(-y.y, y.y, -z.y * z.y)),
+                       fmaf(-x.z, x.z, fmaf(-y.z, y.z, -z.z * z.z)),
+                       fmaf(-x.w, x.w, fmaf(-y.w, y.w, -z.w * z.w)));
+}
 
 // Affinity workgroup order, computed by one more workgroup (kEmRows * kEmWaves
 // threads) of an affinity launch, for the launch after it, in `lds` (the
@@ -660,7 +671,7 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
     static_assert(UNR == kCullCols, "one loop iteration per column group");
     static_assert(kMfPiece / kCullCols <= 32, "group mask of a piece fits 32 bits");
     typedef float f4 __attribute__((ext_vector_type(4)));
-    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][4][kMfStageLd];
+    __shared__ __attribute__((aligned(16))) float stage[kEmWaves][kMfStageRows][kMfStageLd];
     static_assert(sizeof(stage) >= kBalanceLdsBytes, "the order's sort fits the stage");
     if (permNext) {  // workgroup 0 orders the next launch's workgroups
         if (blockIdx.x == 0) {
@@ -863,11 +874,14 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                 *reinterpret_cast<float4*>(&st[0][sc]) = vx;
                 *reinterpret_cast<float4*>(&st[1][sc]) = vy;
                 *reinterpret_cast<float4*>(&st[2][sc]) = vz;
+                if constexpr (EXP)
+                    *reinterpret_cast<float4*>(&st[4][sc]) = neg_sq_norm4(vx, vy, vz);
             } else if (EXP) {
                 const float4 far = make_float4(kExpFar, kExpFar, kExpFar, kExpFar);
                 *reinterpret_cast<float4*>(&st[0][sc]) = far;
                 *reinterpret_cast<float4*>(&st[1][sc]) = far;
                 *reinterpret_cast<float4*>(&st[2][sc]) = far;
+                *reinterpret_cast<float4*>(&st[4][sc]) = neg_sq_norm4(far, far, far);
             }
         }
         wave_lds_sync();
@@ -889,17 +903,11 @@ emicp_affinity_mfma_kernel(const EmRegDesc* __restrict__ regs, int nregs, float 
                 }
                 // A: component (lane >> 4) of column c + (lane & 15); the
                 // lane's 4 columns of phase q = lane >> 4: their (w, 1) for
-                // the B/W operand and -|w|^2 for the accumulator init
+                // the B/W operand and -|w|^2 (staged) for the accumulator init
                 const float aop = st[lane >> 4][c + (lane & 15)];
                 const int cq = c + 4 * (lane >> 4);
                 const f4 xq = *reinterpret_cast<const f4*>(&st[j][cq]);
-                const f4 qx = *reinterpret_cast<const f4*>(&st[0][cq]);
-                const f4 qy = *reinterpret_cast<const f4*>(&st[1][cq]);
-                const f4 qz = *reinterpret_cast<const f4*>(&st[2][cq]);
-                f4 nw2;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)  // -|w|^2, the signs on the FMA operands
-                    nw2[v] = fmaf(-qx[v], qx[v], fmaf(-qy[v], qy[v], -qz[v] * qz[v]));
+                const f4 nw2 = *reinterpret_cast<const f4*>(&st[4][cq]);
 #pragma unroll
                 for (int sb = 0; sb < 4; ++sb) {
                     const f4 ex = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop[sb], nw2, 0, 0, 0);
