@@ -35,11 +35,12 @@ FLOPS_PER_PAIR = 18           # 3 sub, 1 mul + 2 fma (d^2), 1 scale, 2 (C fma), 
 # difference), 3 (d^2), 1 (C fma) on the VALU, 1 v_exp_f32 and one
 # v_mfma_f32_4x4x1f32 (B and W over the column dimension); plain difference
 # (sigma^2 >= 2e-3): 7 VALU; expanded form: 1/4 v_mfma_f32_16x16x4f32 (the
-# exponents of 16 rows x 16 columns), the same exp and 4x4x1 product, C and
-# the columns' -|w|^2 (2 VALU).
+# exponents of 16 rows x 16 columns), the same exp and 4x4x1 product and C
+# (1 VALU; the columns' -|w|^2 is formed once per staged piece, 0.05 per 64
+# pairs).
 VALU_PER_PAIR = 10
 VALU_PER_PAIR_PLAIN = 7        # sigma^2 >= 2e-3: plain difference
-VALU_PER_PAIR_EXPANDED = 2
+VALU_PER_PAIR_EXPANDED = 1
 COMP_BELOW_SIGMA2 = 2e-3
 METRIC = "registrations/s batched vs plain, N=M=10k FP32 @ rot err<=1e-4 rad; fwd+bwd ms"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
