@@ -122,8 +122,12 @@ __device__ void union_boxes(const float* __restrict__ in, int n, int per, float*
 __global__ void __launch_bounds__(kOrdThreads)
 icp_order_kernel(const float* __restrict__ X, int N, const float* __restrict__ Y, int M,
                  IcpBufs b) {
-    if (blockIdx.x == 0) {
-        order_cloud(X, N, b.Xperm, b.tmp, b.Xs, b.ldx, INFINITY);
+    // one inlined sort for both clouds (workgroup-uniform arguments)
+    const bool tgt = blockIdx.x == 0;
+    order_cloud(tgt ? X : Y, tgt ? N : M, tgt ? b.Xperm : b.Yperm,
+                b.tmp + (tgt ? 0 : 4 * (size_t)N), tgt ? b.Xs : b.Ys, tgt ? b.ldx : M,
+                tgt ? INFINITY : 0.f);
+    if (tgt) {
         for (int i = N + threadIdx.x; i < b.ldx; i += blockDim.x) b.Xperm[i] = N;
         __syncthreads();  // the boxes read the ordered target written above
         group_boxes<kIcpGroup>(b.Xs, b.ldx, N, b.gbox);
@@ -133,7 +137,6 @@ icp_order_kernel(const float* __restrict__ X, int N, const float* __restrict__ Y
         __syncthreads();
         union_boxes(b.sbox, supers, kIcpHyper, b.hbox);
     } else {
-        order_cloud(Y, M, b.Yperm, b.tmp + 4 * (size_t)N, b.Ys, M, 0.f);
         for (int r = threadIdx.x; r < M; r += blockDim.x) {
             b.key[r] = kIcpNone;
             b.prev[r] = -1;

@@ -72,7 +72,10 @@ def test_hilbert_curve_steps_are_unit():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [1, 2, 63, 1000, 4096, 4097, 10000, 12345])
+# up to 16384 points the sort keeps its keys in registers and LDS; larger
+# clouds take the global-memory passes
+@pytest.mark.parametrize("n", [1, 2, 63, 1000, 4096, 4097, 10000, 12345, 16383, 16384, 16385,
+                               40000])
 def test_order_matches_numpy(n):
     import torch
     from cuda_emicp_softassign_amd import _lib
