@@ -150,7 +150,7 @@ __device__ inline void order_wave_offsets(unsigned* wcnt, unsigned* tot) {
 __device__ inline void order_cloud_lds(const float* __restrict__ P, int n, int* __restrict__ perm,
                                        float* __restrict__ W, int ld, float pad, const float lo[3],
                                        const float scale[3], unsigned* base, unsigned* tot,
-                                       unsigned* wsum, unsigned* wcnt, uint2* kv) {
+                                       unsigned* wsum, unsigned* wcnt, unsigned* hist, uint2* kv) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int ntiles = (n + kOrdKeys - 1) / kOrdKeys;  // workgroup-uniform
     unsigned key[kOrdLdsTiles][kOrdRounds], val[kOrdLdsTiles][kOrdRounds];
@@ -162,18 +162,22 @@ __device__ inline void order_cloud_lds(const float* __restrict__ P, int n, int* 
             key[tt][j] = (i < n) ? order_point_key(P, n, i, lo, scale) : 0u;
             val[tt][j] = (unsigned)i;
         }
+    // the four passes' digit counts in one sweep (a count does not depend on
+    // the keys' order); wcnt starts at zero and each tile leaves it so
+    for (int q = tid; q < 4 * 256; q += kOrdThreads) hist[q] = 0;
+    for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int tt = 0; tt < kOrdLdsTiles; ++tt)
+#pragma unroll
+        for (int j = 0; j < kOrdRounds; ++j)
+            if (val[tt][j] < (unsigned)n)
+#pragma unroll
+                for (int d = 0; d < 4; ++d) atomicAdd(&hist[256 * d + ((key[tt][j] >> (8 * d)) & 255u)], 1u);
+    __syncthreads();
     for (int pass = 0; pass < 4; ++pass) {
         const int sh = 8 * pass;
-        if (tid < 256) tot[tid] = 0;
-        for (int q = tid; q < kOrdWaves * 256; q += kOrdThreads) wcnt[q] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int tt = 0; tt < kOrdLdsTiles; ++tt)
-#pragma unroll
-            for (int j = 0; j < kOrdRounds; ++j)
-                if (val[tt][j] < (unsigned)n) atomicAdd(&tot[(key[tt][j] >> sh) & 255u], 1u);
-        __syncthreads();
-        order_digit_scan(tot, base, wsum);
+        order_digit_scan(hist + 256 * pass, base, wsum);
 #pragma unroll
         for (int tt = 0; tt < kOrdLdsTiles; ++tt) {
             if (tt < ntiles) {
@@ -262,6 +266,7 @@ __device__ inline void order_cloud(const float* __restrict__ P, int n, int* __re
     __shared__ float boxs[kOrdWaves][6];
     __shared__ unsigned base[256], tot[256], wsum[4];
     __shared__ unsigned wcnt[kOrdWaves * 256];
+    __shared__ unsigned hist[4 * 256];
     __shared__ uint2 kv[kOrdLdsMax];
 
     // bounding box of the finite points
@@ -293,7 +298,7 @@ __device__ inline void order_cloud(const float* __restrict__ P, int n, int* __re
         scale[a] = (hi[a] > lo[a]) ? (float)((1 << kOrdBits) - 1) / (hi[a] - lo[a]) : 0.f;
     }
     if (n <= kOrdLdsMax) {  // workgroup-uniform
-        order_cloud_lds(P, n, perm, W, ld, pad, lo, scale, base, tot, wsum, wcnt, kv);
+        order_cloud_lds(P, n, perm, W, ld, pad, lo, scale, base, tot, wsum, wcnt, hist, kv);
         return;
     }
     for (int i = tid; i < n; i += kOrdThreads) {
